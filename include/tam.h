@@ -1,0 +1,181 @@
+/*
+ * tam.h -- C-ABI of the MI355X Tissue-Ablation-MC hot path.
+ *
+ * Plain C, no torch or HIP types.  The Fortran host binds these through
+ * ISO_C_BINDING (tissue-ablation-mc_amd/fortran/tam_bind.f90); Python binds
+ * them through ctypes (tissue-ablation-mc_amd/tam_amd/lib.py).
+ *
+ * Granularity is one call per photon batch / heat call / cycle, not per
+ * photon: the reference's per-packet subroutines (sourcephCO2, tauint1) are
+ * replaced by tam_mc_run, which runs the whole photon loop on the GPU.
+ *
+ * Conventions (same as the reference's arrays, src/iarray.f90:8-12 and
+ * src/3dFD.f90:10): FP64, column-major (x fastest).  "Halo" arrays are
+ * (nx+2)*(ny+2)*(nz+2) with the Fortran lower bound 0 (src/subs.f90:66,72);
+ * "interior" arrays are nx*ny*nz.  Every function returns TAM_OK (0) or a
+ * negative TAM_ERR_* code; tam_last_error() gives the message.  A context is
+ * owned by one host thread and is not thread safe.
+ */
+#ifndef TAM_H
+#define TAM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TAM_ABI_VERSION 1
+
+enum {
+    TAM_OK = 0,
+    TAM_ERR_ARG = -1,            /* bad argument / shape */
+    TAM_ERR_HIP = -2,            /* HIP runtime error */
+    TAM_ERR_NEGATIVE_TEMP = -3,  /* src/3dFD.f90:179-182 (mpi_abort on T < 0) */
+    TAM_ERR_STATE = -4,          /* call out of order (e.g. heat before thermal state) */
+    TAM_ERR_NOMEM = -5,
+    TAM_ERR_NODEVICE = -6
+};
+
+/* beam profiles (sourceph.f90:28-31 is the top-hat disk) */
+enum { TAM_BEAM_TOPHAT = 0, TAM_BEAM_GAUSSIAN = 1 };
+/* thermal property models for the rebuild */
+enum { TAM_THERMAL_WATER = 0,   /* reference: water-content fits, 3dFD.f90:328-345 */
+       TAM_THERMAL_TABLE = 1 }; /* fixed per-material properties */
+/* faces, in the order of tam_mc_stats.escaped */
+enum { TAM_FACE_XLO = 0, TAM_FACE_XHI, TAM_FACE_YLO, TAM_FACE_YHI, TAM_FACE_ZLO, TAM_FACE_ZHI };
+/* fields for tam_download / tam_upload */
+enum {
+    TAM_FIELD_TEMP = 0,      /* halo, K (heat_sim_3D's temp) */
+    TAM_FIELD_JMEAN = 1,     /* interior, W/m^3 after tam_normalise (jmeanGLOBAL) */
+    TAM_FIELD_TALLY = 2,     /* interior, path length per voxel in cm (raw MC tally) */
+    TAM_FIELD_RHOKAP = 3,    /* halo, cm^-1 (iarray rhokap) */
+    TAM_FIELD_TISSUE = 4,    /* interior, Arrhenius integral Omega (iarray tissue) */
+    TAM_FIELD_THRESTIME = 5, /* interior x 3, s (iarray ThresTime) */
+    TAM_FIELD_WATER = 6,     /* interior (Heat WaterContent) */
+    TAM_FIELD_Q = 7,         /* interior, J (Heat Q) */
+    TAM_FIELD_KAPPA = 8,     /* halo, W/m/K */
+    TAM_FIELD_DENSITY = 9,   /* halo, kg/m^3 */
+    TAM_FIELD_HEATCAP = 10,  /* halo, J/kg/K */
+    TAM_FIELD_COEFF = 11,    /* halo, K m^3/J (delt/(rho c)) */
+    TAM_FIELD_MAT = 12       /* interior material ids, as doubles */
+};
+
+typedef struct tam_ctx tam_ctx;
+
+typedef struct {
+    double mua, mus, g, n;       /* cm^-1, cm^-1, HG anisotropy, refractive index */
+} tam_material;
+
+/* Everything the reference keeps in module globals for this path
+ * (constants.f90:12-14, opt_prop.f90:5, 3dFD.f90:5-12). */
+typedef struct {
+    int32_t nx, ny, nz;          /* voxels (constants.f90:12 nxg,nyg,nzg) */
+    double  xmax, ymax, zmax;    /* half extents, cm: the grid spans [0, 2*xmax] (gridset.f90:23-31) */
+    /* heat (initThermalCoeff, 3dFD.f90:249-291) */
+    double  dx_m, dy_m, dz_m;    /* FD node spacing, m */
+    double  delt;                /* time step, s */
+    double  dscale;              /* 0.5 reproduces the factor of 3dFD.f90:128-129; 1.0 = textbook */
+    double  volume_voxel;        /* m^3, MC voxel volume (3dFD.f90:289) */
+    double  mass_voxel;          /* kg (3dFD.f90:290) */
+    double  qvapor;              /* J (3dFD.f90:291) */
+    double  t_ref;               /* K: the device holds T - t_ref in FP32 */
+    int32_t boil;                /* boiling branch (3dFD.f90:169-183) on/off */
+    int32_t robin_top;           /* convective top face (extension) */
+    double  h_top, t_amb;        /* W/m^2/K, K */
+    /* rebuild (setupThermalCoeff, 3dFD.f90:312-361) */
+    int32_t thermal_mode;        /* TAM_THERMAL_* */
+    double  mu_water, mu_protein;/* ch_opt.f90:171-172 */
+    double  ablate_K;            /* ablateTemp + 273 */
+    /* Arrhenius (3dFD.f90:439-446) */
+    double  arr_A, arr_dE, arr_R, arr_Tlo, arr_Thi, arr_thr[3];
+    int32_t damage_every_step;   /* 1: damage integrated inside every heat step; 0: tam_arrhenius only */
+    /* photon source and boundaries (sourceph.f90:7-49) */
+    int32_t beam;                /* TAM_BEAM_* */
+    double  beam_radius;         /* cm: top-hat radius or Gaussian 1/e^2 radius */
+    double  n_ext_zlo, n_ext_zhi;/* index outside the bottom/top faces; <= 0: index matched */
+    int32_t device;              /* HIP device ordinal */
+    int32_t heat_fp64;           /* 1: FP64 temperatures in the reference's operation order
+                                    (needed for the boiling clamp); 0: FP32 T - t_ref */
+} tam_params;
+
+typedef struct {
+    double packets;
+    double absorbed, specular, escaped[6], missed;   /* packet counts by fate */
+    double crossings, interactions;                  /* work counters */
+    double kernel_ms;                                /* device time of the transport kernel */
+    double killed;                                   /* abandoned after 2^24 events (exit guarantee) */
+} tam_mc_stats;
+
+/* Pulse state (3dFD.f90:7-12, :199-214); advanced by tam_heat_run. */
+typedef struct {
+    double time, pulseCount, repetitionCount, repetitionRate_1, realPulseLength, laserOn;
+    int32_t laser_flag, pulsesDone, pulsesToDo, use_pulse;  /* use_pulse 0: laserOn fixed */
+} tam_pulse;
+
+typedef struct {
+    double mc_ms, normalise_ms, heat_ms, arrhenius_ms, rebuild_ms;   /* last call of each */
+    double heat_steps;           /* steps in the last tam_heat_run */
+    double heat_step_ms;         /* mean device time of one stencil launch in the last tam_heat_run */
+} tam_timing;
+
+/* --- lifetime -------------------------------------------------------- */
+/* Replaces alloc_array/zarray (subs.f90:27-77) and the device side of
+ * initThermalCoeff (3dFD.f90:233-309).  Allocates every device array. */
+int  tam_create(const tam_params* p, tam_ctx** out);
+void tam_destroy(tam_ctx* ctx);
+const char* tam_last_error(const tam_ctx* ctx);
+int  tam_abi_version(void);
+
+/* --- optics: init_opt1 + gridset (ch_opt.f90:7-25, gridset.f90:6-46) ----
+ * mat: interior material ids (< nmat <= 255).  rhokap: optional halo array
+ * of mu_t in cm^-1 (the reference's rhokap, used in TAM_THERMAL_WATER mode);
+ * NULL derives mu_t = mua + mus from the table.  Material ids index the
+ * table for albedo, g and n. */
+int tam_set_optics(tam_ctx* ctx, const uint8_t* mat, const tam_material* table, int32_t nmat,
+                   const double* rhokap);
+
+/* --- thermal state: temp/kappa/density/heatcap halo arrays (3dFD.f90:10,
+ * mcpolar.f90:123-130); Q, W interior or NULL (zero / 0.75). */
+int tam_set_thermal(tam_ctx* ctx, const double* temp, const double* kappa, const double* density,
+                    const double* heatcap, const double* Q, const double* W);
+
+/* --- photon loop: sourcephCO2 + tauint1 (+ stokes) for npackets packets,
+ * replacing mcpolar.f90:151-170.  Tallies path length per voxel (FP64) into
+ * the device tally; accumulate=0 clears it first.  Packet p draws from the
+ * Philox4x32-10 stream key=seed, counter=(p, event, stream). */
+int tam_mc_run(tam_ctx* ctx, int64_t npackets, uint64_t seed, uint32_t stream, int32_t accumulate,
+               tam_mc_stats* stats);
+
+/* --- normalisation: jmeanGLOBAL = mu_a * L * scale (mcpolar.f90:173-174);
+ * scale = (getPwr()/81)/(nphotons*volumeVoxel) in the reference.  Also
+ * forms the per-step heat source delt/(rho c) * jmean for the stencil. */
+int tam_normalise(tam_ctx* ctx, double scale);
+
+/* --- heat_sim_3D (3dFD.f90:21-230): `loops` explicit steps.  pulse (may be
+ * NULL: laser on) is advanced exactly as 3dFD.f90:199-214.  With
+ * damage_every_step the Arrhenius integral runs inside each step. */
+int tam_heat_run(tam_ctx* ctx, int32_t loops, tam_pulse* pulse);
+
+/* --- Arrhenius (3dFD.f90:424-466), once, with the given delt and time. */
+int tam_arrhenius(tam_ctx* ctx, double delt, double time);
+
+/* --- setupThermalCoeff (3dFD.f90:312-361): ablation mask, property and
+ * optics rebuild ("ch_opt rebuild"), then the stencil coefficients. */
+int tam_rebuild(tam_ctx* ctx);
+
+/* --- data movement ---------------------------------------------------- */
+int tam_download(tam_ctx* ctx, int32_t field, double* out);
+int tam_upload(tam_ctx* ctx, int32_t field, const double* in);
+
+/* --- diagnostics ------------------------------------------------------ */
+/* FP64 heat content sum(heatcap * mass_voxel * (T - t_ref)) over the interior, J,
+ * and the energy that left through the boundary faces since tam_set_thermal. */
+int tam_energy(tam_ctx* ctx, double* heat_content, double* boundary_out);
+int tam_get_timing(tam_ctx* ctx, tam_timing* out);
+int tam_synchronize(tam_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -101,7 +101,7 @@ void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
 
 double or_u01(uint32_t x)
 {
-    return ((double)(x >> 8) + 0.5) * (1.0 / 16777216.0);
+    return ((double)(x >> 9) + 0.5) * (1.0 / 8388608.0);
 }
 
 /* ================================================= reference MC (mu_s = 0) */

@@ -53,7 +53,7 @@ double or_ran2(or_ran2_t* s);
 
 /* Philox4x32-10 (Salmon et al., SC'11). */
 void   or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
-/* 24-bit uniform in (0,1): ((x >> 8) + 0.5) * 2^-24 -- identical on the GPU. */
+/* 23-bit uniform in (0,1): ((x >> 9) + 0.5) * 2^-23, exact in FP32 -- identical on the GPU. */
 double or_u01(uint32_t x);
 
 /* ------------------------------------------------ reference MC (mu_s = 0) */

@@ -1,0 +1,63 @@
+"""Build libtam.so (HIP, gfx950) in-tree, plus the Fortran host when flang exists.
+
+    python tissue-ablation-mc_amd/build.py            # library + Fortran host
+    python tissue-ablation-mc_amd/build.py --lib-only
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+SOURCES = ["mc.hip", "heat.hip", "heat64.hip", "rebuild.hip", "api.hip"]
+LIB = os.path.join(PKG, "libtam.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+
+HIP_FLAGS = [
+    "-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared", "-Wno-unused-value",
+    # packed FP32 instructions stay off (hipcc emits them on its own otherwise)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I" + os.path.join(ROOT, "include"),
+]
+
+
+def run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}")
+    return out
+
+
+def build_lib():
+    tmp = LIB + ".tmp"
+    run([HIPCC] + HIP_FLAGS + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", tmp])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_fortran():
+    """Fortran drop-in host (mcgrid) over the C-ABI, if flang is present."""
+    fdir = os.path.join(PKG, "fortran")
+    mk = os.path.join(fdir, "Makefile")
+    if not (os.path.exists(FLANG) and os.path.exists(mk)):
+        return None
+    run(["make", "-s", "-C", fdir, f"FLANG={FLANG}"])
+    return os.path.join(fdir, "mcgrid")
+
+
+def main(argv):
+    lib = build_lib()
+    print("built", lib)
+    if "--lib-only" not in argv:
+        exe = build_fortran()
+        if exe:
+            print("built", exe)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

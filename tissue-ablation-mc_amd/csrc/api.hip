@@ -1,0 +1,690 @@
+// api.hip -- the C-ABI of libtam.so (declared in include/tam.h).
+//
+// A tam_ctx owns every device array of one tissue grid, one HIP stream and
+// timing events.  Host buffers cross the boundary only in tam_set_*,
+// tam_upload and tam_download; everything between (transport, normalisation,
+// heat steps, damage, rebuild) stays resident in HBM.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "tam_internal.h"
+
+namespace {
+
+thread_local std::string g_create_err;
+
+int fail(tam_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    else g_create_err = msg;
+    return code;
+}
+
+int hip_fail(tam_ctx* c, const char* what, hipError_t e)
+{
+    return fail(c, TAM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, call)                                        \
+    do {                                                         \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) return hip_fail((ctx), #call, e_); \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(T** p, long n)
+{
+    return hipMalloc((void**)p, sizeof(T) * (size_t)(n > 0 ? n : 1));
+}
+
+struct Dev {   // make the context's device current for the scope of a call
+    int prev = -1;
+    explicit Dev(int d)
+    {
+        hipGetDevice(&prev);
+        if (prev != d) hipSetDevice(d);
+    }
+    ~Dev()
+    {
+        int cur = -1;
+        hipGetDevice(&cur);
+        if (prev >= 0 && cur != prev) hipSetDevice(prev);
+    }
+};
+
+HeatArgs heat_args(tam_ctx* c)
+{
+    HeatArgs a{};
+    a.nx = c->p.nx; a.ny = c->p.ny; a.nz = c->p.nz;
+    a.px = c->px; a.pxy = c->pxy;
+    a.gx = c->gx; a.gy = c->gy; a.gz = c->gz;
+    a.src = c->src; a.robin = c->robin;
+    a.Q = c->Q; a.jmean = c->jmean; a.heatcap = c->heatcap;
+    a.delt = c->p.delt; a.volume_voxel = c->p.volume_voxel; a.mass_voxel = c->p.mass_voxel;
+    a.qvapor = c->p.qvapor;
+    a.dT_boil = (float)(373.0 - c->p.t_ref);
+    a.t_ref = c->p.t_ref;
+    a.omega = c->omega; a.thres = c->thres; a.rhokap = c->rhokap;
+    a.arr_A = c->p.arr_A; a.arr_dE = c->p.arr_dE; a.arr_R = c->p.arr_R;
+    a.arr_Tlo = c->p.arr_Tlo; a.arr_Thi = c->p.arr_Thi;
+    a.thr0 = c->p.arr_thr[0]; a.thr1 = c->p.arr_thr[1]; a.thr2 = c->p.arr_thr[2];
+    a.energy = c->energy; a.flags = c->flags;
+    return a;
+}
+
+RebuildArgs rebuild_args(tam_ctx* c)
+{
+    RebuildArgs a{};
+    a.nx = c->p.nx; a.ny = c->p.ny; a.nz = c->p.nz;
+    a.px = c->px; a.pxy = c->pxy;
+    a.mode = c->p.thermal_mode;
+    a.ablate_K = c->p.ablate_K; a.delt = c->p.delt;
+    a.mu_water = c->p.mu_water; a.mu_protein = c->p.mu_protein;
+    a.qvapor = c->p.qvapor; a.t_ref = c->p.t_ref;
+    a.dT = c->cur;
+    a.rhokap = c->rhokap; a.kappa = c->kappa; a.density = c->density; a.heatcap = c->heatcap;
+    a.coeff = c->coeff; a.W = c->W; a.Q = c->Q; a.mut = c->mut; a.mat = c->mat;
+    a.mask = c->mask; a.air_id = c->air_id;
+    a.T64 = c->fp64 ? c->cur64 : nullptr;
+    return a;
+}
+
+Heat64Args heat64_args(tam_ctx* c)
+{
+    Heat64Args a{};
+    a.nx = c->p.nx; a.ny = c->p.ny; a.nz = c->p.nz;
+    a.px = c->px; a.pxy = c->pxy;
+    a.ax = c->ax; a.ay = c->ay; a.az = c->az;
+    a.kappa = c->kappa; a.heatcap = c->heatcap; a.coeff = c->coeff; a.jmean = c->jmean; a.Q = c->Q;
+    a.delt = c->p.delt; a.volume_voxel = c->p.volume_voxel; a.mass_voxel = c->p.mass_voxel;
+    a.qvapor = c->p.qvapor; a.h_top = c->p.h_top; a.t_amb = c->p.t_amb; a.dz = c->p.dz_m;
+    a.omega = c->omega; a.thres = c->thres;
+    a.arr_A = c->p.arr_A; a.arr_dE = c->p.arr_dE; a.arr_R = c->p.arr_R;
+    a.arr_Tlo = c->p.arr_Tlo; a.arr_Thi = c->p.arr_Thi;
+    a.thr0 = c->p.arr_thr[0]; a.thr1 = c->p.arr_thr[1]; a.thr2 = c->p.arr_thr[2];
+    a.energy = c->energy; a.flags = c->flags;
+    return a;
+}
+
+hipError_t build_coeffs(tam_ctx* c)
+{
+    if (!c->fp64) {
+        CoeffArgs a{};
+        a.nx = c->p.nx; a.ny = c->p.ny; a.nz = c->p.nz;
+        a.px = c->px; a.pxy = c->pxy;
+        a.delt = c->p.delt; a.dscale = c->p.dscale;
+        a.idx2 = 1.0 / (c->p.dx_m * c->p.dx_m);
+        a.idy2 = 1.0 / (c->p.dy_m * c->p.dy_m);
+        a.idz2 = 1.0 / (c->p.dz_m * c->p.dz_m);
+        a.kappa = c->kappa; a.density = c->density; a.heatcap = c->heatcap;
+        a.gx = c->gx; a.gy = c->gy; a.gz = c->gz;
+        a.robin = c->p.robin_top; a.h_top = c->p.h_top; a.t_amb = c->p.t_amb; a.t_ref = c->p.t_ref;
+        a.dz = c->p.dz_m; a.robin_coef = c->robin;
+        return launch_coeffs(a, c->stream);
+    }
+    Coeff64Args a{};
+    a.nx = c->p.nx; a.ny = c->p.ny; a.nz = c->p.nz;
+    a.px = c->px; a.pxy = c->pxy;
+    a.dscale = c->p.dscale;
+    a.idx2 = 1.0 / (c->p.dx_m * c->p.dx_m);
+    a.idy2 = 1.0 / (c->p.dy_m * c->p.dy_m);
+    a.idz2 = 1.0 / (c->p.dz_m * c->p.dz_m);
+    a.kappa = c->kappa; a.density = c->density; a.heatcap = c->heatcap;
+    a.ax = c->ax; a.ay = c->ay; a.az = c->az;
+    return launch_coeffs64(a, c->stream);
+}
+
+float elapsed(tam_ctx* c)
+{
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    return ms;
+}
+
+int check_flags(tam_ctx* c)
+{
+    int f = 0;
+    HIPCHK(c, hipMemcpyAsync(&f, c->flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (f) return fail(c, TAM_ERR_NEGATIVE_TEMP, "negative temperature in the heat step");
+    return TAM_OK;
+}
+
+// src = coeff * jmean (after a property change the per-step source must use
+// the new delt/(rho c), as heat_sim_3D does with coeff(i,j,k)*jtmp)
+__global__ void src_kernel(int nx, int ny, int nz, long px, long pxy, const double* coeff,
+                           const double* jmean, float* src)
+{
+    const long nv = (long)nx * ny * nz;
+    const long v = (long)blockIdx.x * 256 + threadIdx.x;
+    if (v >= nv) return;
+    const int i = (int)(v % nx) + 1;
+    const long r = v / nx;
+    const int j = (int)(r % ny) + 1;
+    const int k = (int)(r / ny) + 1;
+    const long cc = i + px * j + pxy * k;
+    src[cc] = (float)(coeff[cc] * jmean[v]);
+}
+
+int refresh_src(tam_ctx* c)
+{
+    const long nv = c->nvox;
+    hipLaunchKernelGGL(src_kernel, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, c->stream, c->p.nx, c->p.ny,
+                       c->p.nz, c->px, c->pxy, c->coeff, c->jmean, c->src);
+    HIPCHK(c, hipGetLastError());
+    return TAM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tam_abi_version(void) { return TAM_ABI_VERSION; }
+
+const char* tam_last_error(const tam_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+int tam_create(const tam_params* p, tam_ctx** out)
+{
+    if (!p || !out) return fail(nullptr, TAM_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1 || p->nx > 4096 || p->ny > 4096 || p->nz > 4096)
+        return fail(nullptr, TAM_ERR_ARG, "grid dimensions must be in [1, 4096]");
+    if (!(p->xmax > 0 && p->ymax > 0 && p->zmax > 0))
+        return fail(nullptr, TAM_ERR_ARG, "grid extents must be positive");
+    if (!(p->dx_m > 0 && p->dy_m > 0 && p->dz_m > 0 && p->delt > 0))
+        return fail(nullptr, TAM_ERR_ARG, "FD spacing and delt must be positive");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(nullptr, TAM_ERR_NODEVICE, "no HIP device");
+    if (p->device < 0 || p->device >= ndev) return fail(nullptr, TAM_ERR_ARG, "device ordinal out of range");
+    tam_ctx* c = new (std::nothrow) tam_ctx();
+    if (!c) return fail(nullptr, TAM_ERR_NOMEM, "host allocation failed");
+    c->p = *p;
+    c->dev = p->device;
+    Dev d(c->dev);
+    const long nx = p->nx, ny = p->ny, nz = p->nz;
+    c->nvox = nx * ny * nz;
+    c->px = nx + 2;
+    c->pxy = (nx + 2) * (ny + 2);
+    c->nhalo = c->pxy * (nz + 2);
+    c->air_id = 0;
+    c->fp64 = p->heat_fp64 != 0;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    const long H = c->nhalo, V = c->nvox;
+    if (e == hipSuccess) e = dalloc(&c->dT0, H);
+    if (e == hipSuccess) e = dalloc(&c->dT1, H);
+    if (e == hipSuccess) e = dalloc(&c->gx, H);
+    if (e == hipSuccess) e = dalloc(&c->gy, H);
+    if (e == hipSuccess) e = dalloc(&c->gz, H);
+    if (e == hipSuccess) e = dalloc(&c->src, H);
+    if (e == hipSuccess) e = dalloc(&c->robin, nx * ny);
+    if (e == hipSuccess) e = dalloc(&c->kappa, H);
+    if (e == hipSuccess) e = dalloc(&c->density, H);
+    if (e == hipSuccess) e = dalloc(&c->heatcap, H);
+    if (e == hipSuccess) e = dalloc(&c->coeff, H);
+    if (e == hipSuccess) e = dalloc(&c->rhokap, H);
+    if (e == hipSuccess) e = dalloc(&c->mut, V);
+    if (e == hipSuccess) e = dalloc(&c->mat, V);
+    if (e == hipSuccess) e = dalloc(&c->mask, V);
+    if (e == hipSuccess) e = dalloc(&c->mtab, TAM_MAX_MATERIALS + 1);
+    if (e == hipSuccess) e = dalloc(&c->mua_frac, TAM_MAX_MATERIALS + 1);
+    if (e == hipSuccess) e = dalloc(&c->tally, V);
+    if (e == hipSuccess) e = dalloc(&c->jmean, V);
+    if (e == hipSuccess) e = dalloc(&c->Q, V);
+    if (e == hipSuccess) e = dalloc(&c->W, V);
+    if (e == hipSuccess) e = dalloc(&c->omega, V);
+    if (e == hipSuccess) e = dalloc(&c->thres, 3 * V);
+    if (e == hipSuccess) e = dalloc(&c->energy, 4);
+    if (e == hipSuccess) e = dalloc(&c->flags, 4);
+    if (e == hipSuccess) e = dalloc(&c->counters, TAM_CNT_N);
+    if (c->fp64) {
+        if (e == hipSuccess) e = dalloc(&c->T64a, H);
+        if (e == hipSuccess) e = dalloc(&c->T64b, H);
+        if (e == hipSuccess) e = dalloc(&c->ax, H);
+        if (e == hipSuccess) e = dalloc(&c->ay, H);
+        if (e == hipSuccess) e = dalloc(&c->az, H);
+    }
+    if (e != hipSuccess) {
+        std::string m = std::string("device allocation: ") + hipGetErrorString(e);
+        tam_destroy(c);
+        return fail(nullptr, e == hipErrorOutOfMemory ? TAM_ERR_NOMEM : TAM_ERR_HIP, m);
+    }
+    hipStream_t s = c->stream;
+    launch_fill_f32(c->dT0, H, 0.f, s);
+    launch_fill_f32(c->dT1, H, 0.f, s);
+    launch_fill_f32(c->gx, H, 0.f, s);
+    launch_fill_f32(c->gy, H, 0.f, s);
+    launch_fill_f32(c->gz, H, 0.f, s);
+    launch_fill_f32(c->src, H, 0.f, s);
+    launch_fill_f32((float*)c->robin, 2 * nx * ny, 0.f, s);
+    launch_fill_f64(c->rhokap, H, 0.0, s);
+    launch_fill_f64(c->tally, V, 0.0, s);
+    launch_fill_f64(c->jmean, V, 0.0, s);
+    launch_fill_f64(c->Q, V, 0.0, s);
+    launch_fill_f64(c->W, V, 0.75, s);
+    launch_fill_f64(c->omega, V, 0.0, s);
+    launch_fill_f64(c->thres, 3 * V, 0.0, s);
+    launch_fill_f64(c->energy, 4, 0.0, s);
+    launch_fill_f32((float*)c->flags, 4, 0.f, s);
+    launch_fill_f32(c->mut, V, 0.f, s);
+    e = hipMemsetAsync(c->mat, 0, (size_t)V, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) {
+        std::string m = std::string("device init: ") + hipGetErrorString(e);
+        tam_destroy(c);
+        return fail(nullptr, TAM_ERR_HIP, m);
+    }
+    c->cur = c->dT0;
+    c->nxt = c->dT1;
+    c->cur64 = c->T64a;
+    c->nxt64 = c->T64b;
+    *out = c;
+    return TAM_OK;
+}
+
+void tam_destroy(tam_ctx* c)
+{
+    if (!c) return;
+    Dev d(c->dev);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    void* ptrs[] = {c->dT0, c->dT1, c->gx, c->gy, c->gz, c->src, c->robin, c->kappa, c->density,
+                    c->heatcap, c->coeff, c->rhokap, c->mut, c->mat, c->mask, c->mtab, c->mua_frac,
+                    c->tally, c->jmean, c->Q, c->W, c->omega, c->thres, c->energy, c->flags,
+                    c->counters, c->T64a, c->T64b, c->ax, c->ay, c->az};
+    for (void* q : ptrs)
+        if (q) hipFree(q);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int tam_set_optics(tam_ctx* c, const uint8_t* mat, const tam_material* table, int32_t nmat,
+                   const double* rhokap)
+{
+    if (!c) return TAM_ERR_ARG;
+    if (!mat || !table || nmat < 1 || nmat > TAM_MAX_MATERIALS - 1)
+        return fail(c, TAM_ERR_ARG, "need 1..254 materials and a material grid");
+    Dev d(c->dev);
+    const long V = c->nvox, H = c->nhalo;
+    for (long v = 0; v < V; ++v)
+        if (mat[v] >= nmat) return fail(c, TAM_ERR_ARG, "material id out of range");
+    MatEntry tab[TAM_MAX_MATERIALS + 1];
+    double frac[TAM_MAX_MATERIALS + 1];
+    std::memset(tab, 0, sizeof(tab));
+    std::memset(frac, 0, sizeof(frac));
+    for (int m = 0; m < nmat; ++m) {
+        const tam_material& t = table[m];
+        if (!(t.mua >= 0 && t.mus >= 0 && t.g > -1 && t.g < 1 && t.n > 0))
+            return fail(c, TAM_ERR_ARG, "material properties out of range");
+        const double mt = t.mua + t.mus;
+        if (c->p.thermal_mode == TAM_THERMAL_TABLE && !rhokap && mt <= 0.01)
+            return fail(c, TAM_ERR_ARG, "tissue materials need mu_t > 0.01 cm^-1 (the removal threshold)");
+        tab[m].albedo = mt > 0 ? (float)(t.mus / mt) : 0.f;
+        tab[m].g = (float)t.g;
+        tab[m].n = (float)t.n;
+        tab[m].mua = (float)t.mua;
+        frac[m] = mt > 0 ? t.mua / mt : 0.0;
+        c->host_tab[m] = t;
+    }
+    c->air_id = nmat;                     // removed voxels: no absorption, no scattering, n = 1
+    tab[nmat].albedo = 0.f; tab[nmat].g = 0.f; tab[nmat].n = 1.f; tab[nmat].mua = 0.f;
+    frac[nmat] = 0.0;
+    c->host_tab[nmat] = tam_material{0, 0, 0, 1};
+    c->nmat = nmat;
+    std::vector<double> rk;
+    if (!rhokap) {
+        rk.assign((size_t)H, 0.0);
+        for (int k = 1; k <= c->p.nz; ++k)
+            for (int j = 1; j <= c->p.ny; ++j)
+                for (int i = 1; i <= c->p.nx; ++i) {
+                    const long v = (i - 1) + (long)c->p.nx * ((j - 1) + (long)c->p.ny * (k - 1));
+                    rk[(size_t)(i + c->px * j + c->pxy * k)] = table[mat[v]].mua + table[mat[v]].mus;
+                }
+        rhokap = rk.data();
+    }
+    std::vector<float> mut((size_t)V);
+    for (int k = 1; k <= c->p.nz; ++k)
+        for (int j = 1; j <= c->p.ny; ++j)
+            for (int i = 1; i <= c->p.nx; ++i) {
+                const long v = (i - 1) + (long)c->p.nx * ((j - 1) + (long)c->p.ny * (k - 1));
+                mut[(size_t)v] = (float)rhokap[i + c->px * j + c->pxy * k];
+            }
+    HIPCHK(c, hipMemcpyAsync(c->mat, mat, (size_t)V, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->mtab, tab, sizeof(tab), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->mua_frac, frac, sizeof(frac), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->rhokap, rhokap, sizeof(double) * (size_t)H, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->mut, mut.data(), sizeof(float) * (size_t)V, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->optics_set = true;
+    return TAM_OK;
+}
+
+int tam_set_thermal(tam_ctx* c, const double* temp, const double* kappa, const double* density,
+                    const double* heatcap, const double* Q, const double* W)
+{
+    if (!c) return TAM_ERR_ARG;
+    if (!temp || !kappa || !density || !heatcap) return fail(c, TAM_ERR_ARG, "null thermal array");
+    Dev d(c->dev);
+    const long H = c->nhalo, V = c->nvox;
+    std::vector<float> dT((size_t)H);
+    std::vector<double> coeff((size_t)H, 0.0);
+    for (long i = 0; i < H; ++i) {
+        if (!(density[i] > 0 && heatcap[i] > 0 && kappa[i] >= 0))
+            return fail(c, TAM_ERR_ARG, "thermal properties must be positive");
+        dT[(size_t)i] = (float)(temp[i] - c->p.t_ref);
+    }
+    for (int k = 1; k <= c->p.nz; ++k)
+        for (int j = 1; j <= c->p.ny; ++j)
+            for (int i = 1; i <= c->p.nx; ++i) {
+                const long h = i + c->px * j + c->pxy * k;
+                coeff[(size_t)h] = c->p.delt / (density[h] * heatcap[h]);
+            }
+    const size_t hb = sizeof(double) * (size_t)H;
+    HIPCHK(c, hipMemcpyAsync(c->dT0, dT.data(), sizeof(float) * (size_t)H, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->dT1, dT.data(), sizeof(float) * (size_t)H, hipMemcpyHostToDevice, c->stream));
+    if (c->fp64) {
+        HIPCHK(c, hipMemcpyAsync(c->T64a, temp, hb, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->T64b, temp, hb, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->kappa, kappa, hb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->density, density, hb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->heatcap, heatcap, hb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->coeff, coeff.data(), hb, hipMemcpyHostToDevice, c->stream));
+    if (Q) HIPCHK(c, hipMemcpyAsync(c->Q, Q, sizeof(double) * (size_t)V, hipMemcpyHostToDevice, c->stream));
+    else HIPCHK(c, launch_fill_f64(c->Q, V, 0.0, c->stream));
+    if (W) HIPCHK(c, hipMemcpyAsync(c->W, W, sizeof(double) * (size_t)V, hipMemcpyHostToDevice, c->stream));
+    else HIPCHK(c, launch_fill_f64(c->W, V, 0.75, c->stream));
+    HIPCHK(c, launch_fill_f64(c->energy, 4, 0.0, c->stream));
+    HIPCHK(c, launch_fill_f32((float*)c->flags, 4, 0.f, c->stream));
+    c->cur = c->dT0;
+    c->nxt = c->dT1;
+    c->cur64 = c->T64a;
+    c->nxt64 = c->T64b;
+    HIPCHK(c, build_coeffs(c));
+    HIPCHK(c, refresh_src(c) == TAM_OK ? hipSuccess : hipErrorUnknown);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->thermal_set = true;
+    return TAM_OK;
+}
+
+int tam_mc_run(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream, int32_t accumulate,
+               tam_mc_stats* stats)
+{
+    if (!c) return TAM_ERR_ARG;
+    if (npackets < 0) return fail(c, TAM_ERR_ARG, "negative packet count");
+    if (!c->optics_set) return fail(c, TAM_ERR_STATE, "tam_set_optics must come first");
+    Dev d(c->dev);
+    if (!accumulate) HIPCHK(c, launch_fill_f64(c->tally, c->nvox, 0.0, c->stream));
+    HIPCHK(c, launch_fill_u64(c->counters, TAM_CNT_N, c->stream));
+    McArgs a{};
+    a.nx = c->p.nx; a.ny = c->p.ny; a.nz = c->p.nz;
+    a.dx = (float)(2.0 * c->p.xmax / c->p.nx);
+    a.dy = (float)(2.0 * c->p.ymax / c->p.ny);
+    a.dz = (float)(2.0 * c->p.zmax / c->p.nz);
+    a.cx = (float)c->p.xmax;
+    a.cy = (float)c->p.ymax;
+    a.radius = (float)c->p.beam_radius;
+    a.beam = c->p.beam;
+    a.n_zlo = (float)c->p.n_ext_zlo;
+    a.n_zhi = (float)c->p.n_ext_zhi;
+    a.mut = c->mut; a.mat = c->mat; a.mtab = c->mtab;
+    a.tally = c->tally; a.counters = c->counters;
+    a.p0 = 0; a.npk = (uint64_t)npackets;
+    a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32); a.stream = stream;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, launch_mc(a, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    unsigned long long cnt[TAM_CNT_N];
+    HIPCHK(c, hipMemcpyAsync(cnt, c->counters, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->timing.mc_ms = elapsed(c);
+    c->src_valid = false;
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->packets = (double)npackets;
+        stats->absorbed = (double)cnt[TAM_CNT_ABSORBED];
+        stats->specular = (double)cnt[TAM_CNT_SPECULAR];
+        for (int f = 0; f < 6; ++f) stats->escaped[f] = (double)cnt[TAM_CNT_ESC0 + f];
+        stats->missed = (double)cnt[TAM_CNT_MISSED];
+        stats->crossings = (double)cnt[TAM_CNT_CROSS];
+        stats->interactions = (double)cnt[TAM_CNT_INTER];
+        stats->killed = (double)cnt[TAM_CNT_KILLED];
+        stats->kernel_ms = c->timing.mc_ms;
+    }
+    return TAM_OK;
+}
+
+int tam_normalise(tam_ctx* c, double scale)
+{
+    if (!c) return TAM_ERR_ARG;
+    if (!c->optics_set || !c->thermal_set) return fail(c, TAM_ERR_STATE, "optics and thermal state must be set");
+    Dev d(c->dev);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, launch_normalise(c->p.nx, c->p.ny, c->p.nz, c->px, c->pxy, c->tally, c->rhokap, c->mat,
+                               c->mua_frac, 1, scale, c->coeff, c->jmean, c->src, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->timing.normalise_ms = elapsed(c);
+    return TAM_OK;
+}
+
+int tam_heat_run(tam_ctx* c, int32_t loops, tam_pulse* pulse)
+{
+    if (!c) return TAM_ERR_ARG;
+    if (loops < 0) return fail(c, TAM_ERR_ARG, "negative loop count");
+    if (!c->thermal_set) return fail(c, TAM_ERR_STATE, "tam_set_thermal must come first");
+    Dev d(c->dev);
+    const HeatArgs a = heat_args(c);
+    const Heat64Args a64 = heat64_args(c);
+    const int boil = c->p.boil != 0, robin = c->p.robin_top != 0, dmg = c->p.damage_every_step != 0;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int p = 0; p < loops; ++p) {
+        const double lon = (pulse && pulse->use_pulse) ? pulse->laserOn : 1.0;
+        double t_after = (pulse ? pulse->time : c->time) + c->p.delt;
+        if (c->fp64) {
+            HIPCHK(c, launch_heat64_step(a64, c->cur64, c->nxt64, lon, t_after, boil, robin, dmg, 1, c->stream));
+            std::swap(c->cur64, c->nxt64);
+        } else {
+            HIPCHK(c, launch_heat_step(a, c->cur, c->nxt, (float)lon, t_after, boil, robin, dmg, 1, c->stream));
+            std::swap(c->cur, c->nxt);
+        }
+        if (pulse) {
+            if (pulse->use_pulse) {                         // 3dFD.f90:199-210
+                if (pulse->pulseCount >= pulse->realPulseLength && pulse->laser_flag) {
+                    pulse->laser_flag = 0;
+                    pulse->laserOn = 0.;
+                    pulse->pulseCount = 0.;
+                    pulse->pulsesDone += 1;
+                    pulse->repetitionCount = 0.;
+                } else if (pulse->repetitionCount >= pulse->repetitionRate_1 && !pulse->laser_flag &&
+                           pulse->pulsesDone < pulse->pulsesToDo) {
+                    pulse->laser_flag = 1;
+                    pulse->laserOn = 1.;
+                    pulse->pulseCount = 0.;
+                    pulse->repetitionCount = 0.;
+                }
+            }
+            pulse->pulseCount = pulse->pulseCount + c->p.delt;     // :212-214
+            pulse->repetitionCount = pulse->repetitionCount + c->p.delt;
+            pulse->time = pulse->time + c->p.delt;
+        } else {
+            c->time += c->p.delt;
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    int rc = check_flags(c);
+    c->timing.heat_ms = elapsed(c);
+    c->timing.heat_steps = loops;
+    c->timing.heat_step_ms = loops > 0 ? c->timing.heat_ms / loops : 0.0;
+    return rc;
+}
+
+int tam_arrhenius(tam_ctx* c, double delt, double time)
+{
+    if (!c) return TAM_ERR_ARG;
+    if (!c->thermal_set) return fail(c, TAM_ERR_STATE, "tam_set_thermal must come first");
+    Dev d(c->dev);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, launch_arrhenius(heat_args(c), c->cur, c->fp64 ? c->cur64 : nullptr, delt, time, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->timing.arrhenius_ms = elapsed(c);
+    return TAM_OK;
+}
+
+int tam_rebuild(tam_ctx* c)
+{
+    if (!c) return TAM_ERR_ARG;
+    if (!c->thermal_set || !c->optics_set) return fail(c, TAM_ERR_STATE, "optics and thermal state must be set");
+    Dev d(c->dev);
+    const RebuildArgs a = rebuild_args(c);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (c->p.thermal_mode == TAM_THERMAL_WATER) HIPCHK(c, launch_water_update(a, c->stream));
+    HIPCHK(c, launch_removal_mask(a, c->stream));
+    HIPCHK(c, launch_apply_props(a, c->stream));
+    HIPCHK(c, build_coeffs(c));
+    if (refresh_src(c) != TAM_OK) return TAM_ERR_HIP;
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->timing.rebuild_ms = elapsed(c);
+    return TAM_OK;
+}
+
+int tam_download(tam_ctx* c, int32_t field, double* out)
+{
+    if (!c) return TAM_ERR_ARG;
+    if (!out) return fail(c, TAM_ERR_ARG, "null output");
+    Dev d(c->dev);
+    const long H = c->nhalo, V = c->nvox;
+    const double* src64 = nullptr;
+    long n = 0;
+    switch (field) {
+    case TAM_FIELD_TEMP: {
+        if (c->fp64) {
+            HIPCHK(c, hipMemcpyAsync(out, c->cur64, sizeof(double) * (size_t)H, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            return TAM_OK;
+        }
+        std::vector<float> t((size_t)H);
+        HIPCHK(c, hipMemcpyAsync(t.data(), c->cur, sizeof(float) * (size_t)H, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (long i = 0; i < H; ++i) out[i] = c->p.t_ref + (double)t[(size_t)i];
+        return TAM_OK;
+    }
+    case TAM_FIELD_MAT: {
+        std::vector<uint8_t> m((size_t)V);
+        HIPCHK(c, hipMemcpyAsync(m.data(), c->mat, (size_t)V, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (long i = 0; i < V; ++i) out[i] = m[(size_t)i];
+        return TAM_OK;
+    }
+    case TAM_FIELD_JMEAN: src64 = c->jmean; n = V; break;
+    case TAM_FIELD_TALLY: src64 = c->tally; n = V; break;
+    case TAM_FIELD_RHOKAP: src64 = c->rhokap; n = H; break;
+    case TAM_FIELD_TISSUE: src64 = c->omega; n = V; break;
+    case TAM_FIELD_THRESTIME: src64 = c->thres; n = 3 * V; break;
+    case TAM_FIELD_WATER: src64 = c->W; n = V; break;
+    case TAM_FIELD_Q: src64 = c->Q; n = V; break;
+    case TAM_FIELD_KAPPA: src64 = c->kappa; n = H; break;
+    case TAM_FIELD_DENSITY: src64 = c->density; n = H; break;
+    case TAM_FIELD_HEATCAP: src64 = c->heatcap; n = H; break;
+    case TAM_FIELD_COEFF: src64 = c->coeff; n = H; break;
+    default: return fail(c, TAM_ERR_ARG, "unknown field");
+    }
+    HIPCHK(c, hipMemcpyAsync(out, src64, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TAM_OK;
+}
+
+int tam_upload(tam_ctx* c, int32_t field, const double* in)
+{
+    if (!c) return TAM_ERR_ARG;
+    if (!in) return fail(c, TAM_ERR_ARG, "null input");
+    Dev d(c->dev);
+    const long H = c->nhalo, V = c->nvox;
+    double* dst = nullptr;
+    long n = 0;
+    switch (field) {
+    case TAM_FIELD_TEMP: {
+        std::vector<float> t((size_t)H);
+        for (long i = 0; i < H; ++i) t[(size_t)i] = (float)(in[i] - c->p.t_ref);
+        HIPCHK(c, hipMemcpyAsync(c->dT0, t.data(), sizeof(float) * (size_t)H, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->dT1, t.data(), sizeof(float) * (size_t)H, hipMemcpyHostToDevice, c->stream));
+        if (c->fp64) {
+            HIPCHK(c, hipMemcpyAsync(c->T64a, in, sizeof(double) * (size_t)H, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(c->T64b, in, sizeof(double) * (size_t)H, hipMemcpyHostToDevice, c->stream));
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->cur = c->dT0;
+        c->nxt = c->dT1;
+        c->cur64 = c->T64a;
+        c->nxt64 = c->T64b;
+        return TAM_OK;
+    }
+    case TAM_FIELD_RHOKAP: {
+        std::vector<float> mut((size_t)V);
+        for (int k = 1; k <= c->p.nz; ++k)
+            for (int j = 1; j <= c->p.ny; ++j)
+                for (int i = 1; i <= c->p.nx; ++i)
+                    mut[(size_t)((i - 1) + (long)c->p.nx * ((j - 1) + (long)c->p.ny * (k - 1)))] =
+                        (float)in[i + c->px * j + c->pxy * k];
+        HIPCHK(c, hipMemcpyAsync(c->mut, mut.data(), sizeof(float) * (size_t)V, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->rhokap, in, sizeof(double) * (size_t)H, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return TAM_OK;
+    }
+    case TAM_FIELD_JMEAN: dst = c->jmean; n = V; break;
+    case TAM_FIELD_TALLY: dst = c->tally; n = V; break;
+    case TAM_FIELD_TISSUE: dst = c->omega; n = V; break;
+    case TAM_FIELD_THRESTIME: dst = c->thres; n = 3 * V; break;
+    case TAM_FIELD_WATER: dst = c->W; n = V; break;
+    case TAM_FIELD_Q: dst = c->Q; n = V; break;
+    case TAM_FIELD_COEFF: dst = c->coeff; n = H; break;
+    default: return fail(c, TAM_ERR_ARG, "field cannot be uploaded");
+    }
+    HIPCHK(c, hipMemcpyAsync(dst, in, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    if (field == TAM_FIELD_JMEAN || field == TAM_FIELD_COEFF) {
+        if (refresh_src(c) != TAM_OK) return TAM_ERR_HIP;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TAM_OK;
+}
+
+int tam_energy(tam_ctx* c, double* heat_content, double* boundary_out)
+{
+    if (!c) return TAM_ERR_ARG;
+    Dev d(c->dev);
+    HIPCHK(c, launch_fill_f64(c->energy + 1, 1, 0.0, c->stream));
+    HIPCHK(c, launch_energy(c->p.nx, c->p.ny, c->p.nz, c->px, c->pxy, c->cur, c->fp64 ? c->cur64 : nullptr,
+                            c->p.t_ref, c->heatcap, c->p.mass_voxel, c->energy + 1, c->stream));
+    double e[2];
+    HIPCHK(c, hipMemcpyAsync(e, c->energy, sizeof(e), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (boundary_out) *boundary_out = e[0];
+    if (heat_content) *heat_content = e[1];
+    return TAM_OK;
+}
+
+int tam_get_timing(tam_ctx* c, tam_timing* out)
+{
+    if (!c || !out) return TAM_ERR_ARG;
+    *out = c->timing;
+    return TAM_OK;
+}
+
+int tam_synchronize(tam_ctx* c)
+{
+    if (!c) return TAM_ERR_ARG;
+    Dev d(c->dev);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TAM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,175 @@
+// tam_internal.h -- device data layout, kernel argument blocks and launcher
+// prototypes shared by the HIP translation units of libtam.so.
+//
+// HBM layout (all arrays column-major, x fastest, like the reference):
+//   halo grids   (nx+2)*(ny+2)*(nz+2): FP32 dT ping-pong, FP32 face
+//                coefficients gx/gy/gz, FP32 heat source, FP64 kappa,
+//                density, heatcap, coeff, rhokap
+//   interior     nx*ny*nz: FP32 mu_t, uint8 material id, FP64 path-length
+//                tally, FP64 jmean, FP64 Q, FP64 water, FP64 Omega,
+//                FP64 threshold times (x3)
+// Temperatures live on the device as dT = T - t_ref in FP32: the explicit
+// update adds increments far below the FP32 ulp of an absolute Kelvin value,
+// while relative to t_ref they keep ~1e-7 relative precision.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/tam.h"
+
+#define TAM_MAX_MATERIALS 255
+
+struct MatEntry {          // per material, in device memory
+    float albedo, g, n, mua;
+};
+
+struct McArgs {
+    int nx, ny, nz;
+    float dx, dy, dz;          // voxel edge, cm
+    float cx, cy;              // beam centre, cm (grid coordinates)
+    float radius;              // beam radius, cm
+    int beam;
+    float n_zlo, n_zhi;        // outside indices (<= 0: matched)
+    const float* mut;          // interior mu_t, cm^-1
+    const uint8_t* mat;        // interior material ids
+    const MatEntry* mtab;
+    double* tally;             // interior path length, cm
+    unsigned long long* counters;   // TAM_CNT_*
+    uint64_t p0, npk;
+    uint32_t key0, key1, stream;
+};
+
+enum { TAM_CNT_ABSORBED = 0, TAM_CNT_SPECULAR, TAM_CNT_ESC0, TAM_CNT_MISSED = TAM_CNT_ESC0 + 6,
+       TAM_CNT_CROSS, TAM_CNT_INTER, TAM_CNT_KILLED, TAM_CNT_N };
+
+struct HeatArgs {
+    int nx, ny, nz;
+    long px, pxy;              // halo pitches
+    const float* gx;           // face coefficient between c and c+1 (includes delt)
+    const float* gy;           // ... c and c+px
+    const float* gz;           // ... c and c+pxy
+    const float* src;          // per-step source delt/(rho c) * jmean, K
+    const float2* robin;       // per column (beta, gamma'): dT_ghost = beta*dT_top + gamma'
+    // boiling (3dFD.f90:167-183)
+    double* Q;                 // interior
+    const double* jmean;       // interior, W/m^3
+    const double* heatcap;     // halo
+    double delt, volume_voxel, mass_voxel, qvapor;
+    float dT_boil;             // 373 - t_ref
+    double t_ref;
+    // Arrhenius inside the step
+    double* omega;             // interior
+    double* thres;             // interior x3
+    const double* rhokap;      // halo
+    double arr_A, arr_dE, arr_R, arr_Tlo, arr_Thi, thr0, thr1, thr2;
+    // energy accounting
+    double* energy;            // [0] boundary outflow, J
+    int* flags;                // [0] negative temperature seen
+};
+
+struct Heat64Args {            // FP64 reference-order step (heat64.hip)
+    int nx, ny, nz;
+    long px, pxy;
+    const double* ax;          // a_f (no delt) between c and c+1
+    const double* ay;
+    const double* az;
+    const double* kappa;       // halo (Robin ghost)
+    const double* heatcap;     // halo
+    const double* coeff;       // halo, delt/(rho c)
+    const double* jmean;       // interior, W/m^3
+    double* Q;
+    double delt, volume_voxel, mass_voxel, qvapor;
+    double h_top, t_amb, dz;
+    double* omega; double* thres;
+    double arr_A, arr_dE, arr_R, arr_Tlo, arr_Thi, thr0, thr1, thr2;
+    double* energy;
+    int* flags;
+};
+
+struct Coeff64Args {
+    int nx, ny, nz;
+    long px, pxy;
+    double dscale, idx2, idy2, idz2;
+    const double* kappa; const double* density; const double* heatcap;
+    double* ax; double* ay; double* az;
+};
+
+struct RebuildArgs {
+    int nx, ny, nz;
+    long px, pxy;
+    int mode;                  // TAM_THERMAL_*
+    double ablate_K, delt, mu_water, mu_protein, qvapor, t_ref;
+    const float* dT;           // current temperature (halo), FP32 path
+    const double* T64;         // current temperature (halo), FP64 path (nullptr otherwise)
+    double* rhokap;            // halo
+    double* kappa; double* density; double* heatcap; double* coeff;
+    double* W; const double* Q;
+    float* mut; uint8_t* mat;
+    uint8_t* mask;             // interior scratch
+    int air_id;                // material id given to removed voxels
+};
+
+struct CoeffArgs {
+    int nx, ny, nz;
+    long px, pxy;
+    double delt, dscale, idx2, idy2, idz2;
+    const double* kappa; const double* density; const double* heatcap;
+    float* gx; float* gy; float* gz;
+    int robin; double h_top, t_amb, t_ref, dz;
+    float2* robin_coef;
+};
+
+struct tam_ctx {
+    tam_params p;
+    int dev;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;
+    std::string err;
+    long nvox, nhalo, px, pxy;
+    int nmat, air_id;
+    bool optics_set, thermal_set, src_valid;
+    // device arrays
+    float *dT0, *dT1, *gx, *gy, *gz, *src;
+    float *cur, *nxt;           // current / next temperature buffer (ping-pong)
+    bool fp64;                  // FP64 reference-order heat path
+    double *T64a, *T64b, *cur64, *nxt64, *ax, *ay, *az;
+    float2* robin;
+    double *kappa, *density, *heatcap, *coeff, *rhokap;
+    float* mut;
+    uint8_t *mat, *mask;
+    MatEntry* mtab;
+    double* mua_frac;           // per material mu_a / mu_t (FP64)
+    double *tally, *jmean, *Q, *W, *omega, *thres;
+    double* energy;             // [0] boundary outflow, [1..] reduction scratch
+    int* flags;
+    unsigned long long* counters;
+    double time;                // heat time when no pulse state is passed
+    tam_material host_tab[TAM_MAX_MATERIALS + 1];
+    tam_timing timing;
+};
+
+// ---- launchers (each enqueues on `s`, returns hipError_t) --------------
+hipError_t launch_mc(const McArgs& a, hipStream_t s);
+hipError_t launch_heat_step(const HeatArgs& a, const float* Tin, float* Tout, float lon,
+                            double time_after, int boil, int robin, int damage, int energy,
+                            hipStream_t s);
+hipError_t launch_heat64_step(const Heat64Args& a, const double* Tin, double* Tout, double lon,
+                              double time_after, int boil, int robin, int damage, int energy, hipStream_t s);
+hipError_t launch_coeffs64(const Coeff64Args& a, hipStream_t s);
+hipError_t launch_arrhenius(const HeatArgs& a, const float* T, const double* T64, double delt, double time,
+                            hipStream_t s);
+hipError_t launch_removal_mask(const RebuildArgs& a, hipStream_t s);
+hipError_t launch_apply_props(const RebuildArgs& a, hipStream_t s);
+hipError_t launch_water_update(const RebuildArgs& a, hipStream_t s);
+hipError_t launch_coeffs(const CoeffArgs& a, hipStream_t s);
+hipError_t launch_normalise(int nx, int ny, int nz, long px, long pxy, const double* tally,
+                            const double* rhokap, const uint8_t* mat, const double* mua_tab,
+                            int use_rhokap, double scale, const double* coeff, double* jmean,
+                            float* src, hipStream_t s);
+hipError_t launch_fill_f64(double* p, long n, double v, hipStream_t s);
+hipError_t launch_fill_f32(float* p, long n, float v, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long* p, long n, hipStream_t s);
+hipError_t launch_energy(int nx, int ny, int nz, long px, long pxy, const float* dT, const double* T64,
+                         double t_ref, const double* heatcap, double mass_voxel, double* out, hipStream_t s);

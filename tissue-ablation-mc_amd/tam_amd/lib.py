@@ -1,0 +1,227 @@
+"""ctypes binding of libtam.so (the C-ABI in include/tam.h).
+
+The HIP library is the only implementation: if libtam.so is missing this
+module raises, it never falls back to a CPU path.
+
+Arrays are numpy, C-order with shape (nz, ny, nx) for interior fields and
+(nz+2, ny+2, nx+2) for halo fields -- the same memory layout as the
+reference's column-major (x, y, z) Fortran arrays (src/iarray.f90:8-12).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libtam.so")
+
+TAM_OK = 0
+BEAM_TOPHAT, BEAM_GAUSSIAN = 0, 1
+THERMAL_WATER, THERMAL_TABLE = 0, 1
+FACE_XLO, FACE_XHI, FACE_YLO, FACE_YHI, FACE_ZLO, FACE_ZHI = range(6)
+(FIELD_TEMP, FIELD_JMEAN, FIELD_TALLY, FIELD_RHOKAP, FIELD_TISSUE, FIELD_THRESTIME, FIELD_WATER,
+ FIELD_Q, FIELD_KAPPA, FIELD_DENSITY, FIELD_HEATCAP, FIELD_COEFF, FIELD_MAT) = range(13)
+HALO_FIELDS = {FIELD_TEMP, FIELD_RHOKAP, FIELD_KAPPA, FIELD_DENSITY, FIELD_HEATCAP, FIELD_COEFF}
+
+EXPORTED = ("tam_abi_version", "tam_create", "tam_destroy", "tam_last_error", "tam_set_optics",
+            "tam_set_thermal", "tam_mc_run", "tam_normalise", "tam_heat_run", "tam_arrhenius",
+            "tam_rebuild", "tam_download", "tam_upload", "tam_energy", "tam_get_timing",
+            "tam_synchronize")
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("xmax", C.c_double), ("ymax", C.c_double), ("zmax", C.c_double),
+        ("dx_m", C.c_double), ("dy_m", C.c_double), ("dz_m", C.c_double),
+        ("delt", C.c_double), ("dscale", C.c_double),
+        ("volume_voxel", C.c_double), ("mass_voxel", C.c_double), ("qvapor", C.c_double),
+        ("t_ref", C.c_double), ("boil", C.c_int32), ("robin_top", C.c_int32),
+        ("h_top", C.c_double), ("t_amb", C.c_double),
+        ("thermal_mode", C.c_int32), ("mu_water", C.c_double), ("mu_protein", C.c_double),
+        ("ablate_K", C.c_double),
+        ("arr_A", C.c_double), ("arr_dE", C.c_double), ("arr_R", C.c_double),
+        ("arr_Tlo", C.c_double), ("arr_Thi", C.c_double), ("arr_thr", C.c_double * 3),
+        ("damage_every_step", C.c_int32),
+        ("beam", C.c_int32), ("beam_radius", C.c_double),
+        ("n_ext_zlo", C.c_double), ("n_ext_zhi", C.c_double),
+        ("device", C.c_int32), ("heat_fp64", C.c_int32),
+    ]
+
+
+class Material(C.Structure):
+    _fields_ = [("mua", C.c_double), ("mus", C.c_double), ("g", C.c_double), ("n", C.c_double)]
+
+
+class MCStats(C.Structure):
+    _fields_ = [("packets", C.c_double), ("absorbed", C.c_double), ("specular", C.c_double),
+                ("escaped", C.c_double * 6), ("missed", C.c_double), ("crossings", C.c_double),
+                ("interactions", C.c_double), ("kernel_ms", C.c_double), ("killed", C.c_double)]
+
+    def fractions(self):
+        n = self.packets
+        return {"absorbed": self.absorbed / n, "specular": self.specular / n,
+                "diffuse_reflected": self.escaped[FACE_ZHI] / n,
+                "transmitted": self.escaped[FACE_ZLO] / n,
+                "lateral": sum(self.escaped[f] for f in range(4)) / n,
+                "missed": self.missed / n}
+
+
+class Pulse(C.Structure):
+    _fields_ = [("time", C.c_double), ("pulseCount", C.c_double), ("repetitionCount", C.c_double),
+                ("repetitionRate_1", C.c_double), ("realPulseLength", C.c_double),
+                ("laserOn", C.c_double), ("laser_flag", C.c_int32), ("pulsesDone", C.c_int32),
+                ("pulsesToDo", C.c_int32), ("use_pulse", C.c_int32)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("mc_ms", C.c_double), ("normalise_ms", C.c_double), ("heat_ms", C.c_double),
+                ("arrhenius_ms", C.c_double), ("rebuild_ms", C.c_double),
+                ("heat_steps", C.c_double), ("heat_step_ms", C.c_double)]
+
+
+class TamError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tam error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load libtam.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built: run __graft_entry__.build() "
+                          "(or python tissue-ablation-mc_amd/build.py)")
+    L = C.CDLL(LIB_PATH)
+    vp, dp = C.c_void_p, C.c_void_p
+    L.tam_abi_version.restype = C.c_int
+    L.tam_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+    L.tam_destroy.argtypes = [vp]
+    L.tam_destroy.restype = None
+    L.tam_last_error.argtypes = [vp]
+    L.tam_last_error.restype = C.c_char_p
+    L.tam_set_optics.argtypes = [vp, vp, C.POINTER(Material), C.c_int32, dp]
+    L.tam_set_thermal.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.tam_mc_run.argtypes = [vp, C.c_int64, C.c_uint64, C.c_uint32, C.c_int32, C.POINTER(MCStats)]
+    L.tam_normalise.argtypes = [vp, C.c_double]
+    L.tam_heat_run.argtypes = [vp, C.c_int32, C.POINTER(Pulse)]
+    L.tam_arrhenius.argtypes = [vp, C.c_double, C.c_double]
+    L.tam_rebuild.argtypes = [vp]
+    L.tam_download.argtypes = [vp, C.c_int32, dp]
+    L.tam_upload.argtypes = [vp, C.c_int32, dp]
+    L.tam_energy.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.tam_get_timing.argtypes = [vp, C.POINTER(Timing)]
+    L.tam_synchronize.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, shape):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.shape != shape:
+        a = a.reshape(shape)
+    return a
+
+
+class TissueGrid:
+    """One tissue grid resident on one GPU (a tam_ctx)."""
+
+    def __init__(self, params: Params):
+        self.L = load()
+        self.p = params
+        self.shape = (params.nz, params.ny, params.nx)
+        self.hshape = (params.nz + 2, params.ny + 2, params.nx + 2)
+        h = C.c_void_p()
+        rc = self.L.tam_create(C.byref(params), C.byref(h))
+        if rc != TAM_OK:
+            raise TamError(rc, self.L.tam_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.tam_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != TAM_OK:
+            raise TamError(rc, self.L.tam_last_error(self.h).decode())
+
+    def set_optics(self, mat, table, rhokap=None):
+        mat = np.ascontiguousarray(mat, dtype=np.uint8).reshape(self.shape)
+        tab = (Material * len(table))(*[Material(*map(float, t)) for t in table])
+        rk = None if rhokap is None else _f64(rhokap, self.hshape)
+        self._keep_optics = (mat, rk)
+        self._chk(self.L.tam_set_optics(self.h, _ptr(mat), tab, len(table), _ptr(rk)))
+
+    def set_thermal(self, temp, kappa, density, heatcap, Q=None, W=None):
+        t, k, d, c = (_f64(x, self.hshape) for x in (temp, kappa, density, heatcap))
+        q = None if Q is None else _f64(Q, self.shape)
+        w = None if W is None else _f64(W, self.shape)
+        self._chk(self.L.tam_set_thermal(self.h, _ptr(t), _ptr(k), _ptr(d), _ptr(c), _ptr(q), _ptr(w)))
+
+    def mc_run(self, npackets, seed, stream=0, accumulate=False):
+        st = MCStats()
+        self._chk(self.L.tam_mc_run(self.h, int(npackets), int(seed), int(stream), int(bool(accumulate)),
+                                    C.byref(st)))
+        return st
+
+    def normalise(self, scale):
+        self._chk(self.L.tam_normalise(self.h, float(scale)))
+
+    def heat_run(self, loops, pulse=None):
+        self._chk(self.L.tam_heat_run(self.h, int(loops), None if pulse is None else C.byref(pulse)))
+
+    def arrhenius(self, delt, time):
+        self._chk(self.L.tam_arrhenius(self.h, float(delt), float(time)))
+
+    def rebuild(self):
+        self._chk(self.L.tam_rebuild(self.h))
+
+    def download(self, field):
+        if field in HALO_FIELDS:
+            shape = self.hshape
+        elif field == FIELD_THRESTIME:
+            shape = (3,) + self.shape
+        else:
+            shape = self.shape
+        out = np.empty(shape, dtype=np.float64)
+        self._chk(self.L.tam_download(self.h, field, _ptr(out)))
+        return out
+
+    def upload(self, field, arr):
+        if field in HALO_FIELDS:
+            shape = self.hshape
+        elif field == FIELD_THRESTIME:
+            shape = (3,) + self.shape
+        else:
+            shape = self.shape
+        a = _f64(arr, shape)
+        self._chk(self.L.tam_upload(self.h, field, _ptr(a)))
+
+    def energy(self):
+        hc, bo = C.c_double(), C.c_double()
+        self._chk(self.L.tam_energy(self.h, C.byref(hc), C.byref(bo)))
+        return hc.value, bo.value
+
+    def timing(self):
+        t = Timing()
+        self._chk(self.L.tam_get_timing(self.h, C.byref(t)))
+        return t
+
+    def synchronize(self):
+        self._chk(self.L.tam_synchronize(self.h))
