@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Coupled photon-transport / heat-diffusion cycle throughput on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): three-layer
+skin on a 200^3 grid over 2x2x1 cm, 1e8 photon packets from a Gaussian beam
+(1/e^2 radius 0.15 cm) per cycle, 10,000 FP32 stencil steps per cycle with
+the FP64 Arrhenius integral inside every step, ablation above 100 C and the
+optics/property rebuild, repeated over coupling cycles.  A "step" of this
+benchmark is one whole coupling cycle; --steps K times K cycles after
+--warmup W cycles (the tissue is reset to its initial state in between).
+
+Prints one JSON line.  `value` is whole-job photon packets per second over
+the timed cycles (stencil work included); the FD voxel-updates per second,
+the cycle time and the per-kernel rates (HIP-event times on the library's
+stream) are reported beside it, with the roofline of the dominant kernel and
+the CPU baseline (the FP64 oracle, OpenMP, timed on this host on a bounded
+sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tissue-ablation-mc_amd"))
+
+HBM_PEAK_GBS = 8000.0                 # MI355X HBM3E
+LANE_OPS_PEAK = 256 * 64 * 2.4e9      # 256 CU x 64 lanes x 2.4 GHz (non-packed FP32 lane-ops/s)
+STENCIL_BYTES = 13.0                  # algorithmic bytes per voxel-update (T r/w, source, material id)
+MC_OPS_CROSS, MC_OPS_INTER = 16.0, 64.0   # algorithmic FP32 lane-ops per crossing / interaction
+
+
+def cpu_baseline(prob, budget_s):
+    """FP64 oracle on this host: MC packets/s and stencil voxel-updates/s on a
+    bounded sample, combined into the same cycle's packets/s."""
+    import ctypes as C
+
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as O
+    from tam_amd import lib
+    p = prob.params
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    mc = O.MCProblem(p.nx, p.ny, p.nz, p.xmax, p.ymax, p.zmax, prob.mat, prob.table,
+                     n_ext=(0, 0, 0, 0, p.n_ext_zlo, p.n_ext_zhi), beam=p.beam, radius=p.beam_radius,
+                     rng=0, seed=5)
+    npk, t_mc, batch = 0, 0.0, 20000
+    while t_mc < 0.6 * budget_s and npk < 50_000_000:
+        t0 = time.perf_counter()
+        mc.run(batch, p0=npk, nthreads=threads)
+        t_mc += time.perf_counter() - t0
+        npk += batch
+        batch = min(batch * 2, 2_000_000)
+    pk_rate = npk / t_mc
+    # heat: the oracle's reference-order FP64 step on the full grid
+    hs = (p.nz + 2, p.ny + 2, p.nx + 2)
+    T = np.ascontiguousarray(prob.temp.copy())
+    coeff = np.zeros(hs)
+    coeff[1:-1, 1:-1, 1:-1] = p.delt / (prob.density * prob.heatcap)[1:-1, 1:-1, 1:-1]
+    jm = np.ascontiguousarray(np.full((p.nz, p.ny, p.nx), 1e6))
+    Q = np.zeros((p.nz, p.ny, p.nx))
+    arrs = [np.ascontiguousarray(a) for a in (prob.kappa, prob.density, prob.heatcap, coeff)]
+    h = O.Heat()
+    h.nx, h.ny, h.nz = p.nx, p.ny, p.nz
+    h.dx, h.dy, h.dz, h.delt, h.dscale = p.dx_m, p.dy_m, p.dz_m, p.delt, p.dscale
+    h.temp = T.ctypes.data
+    h.kappa, h.density, h.heatcap, h.coeff = [a.ctypes.data for a in arrs]
+    h.Q, h.jmean = Q.ctypes.data, jm.ctypes.data
+    h.volumeVoxel, h.massVoxel, h.QVapor = p.volume_voxel, p.mass_voxel, p.qvapor
+    h.robin_top, h.h_top, h.T_amb = p.robin_top, p.h_top, p.t_amb
+    steps, t_fd = 0, 0.0
+    while t_fd < 0.4 * budget_s and steps < 1000:
+        t0 = time.perf_counter()
+        O.lib().or_heat_steps(C.byref(h), 1, None)
+        t_fd += time.perf_counter() - t0
+        steps += 1
+    vu_rate = steps * p.nx * p.ny * p.nz / t_fd
+    cyc = prob.packets / pk_rate + prob.steps * p.nx * p.ny * p.nz / vu_rate
+    return {"value": prob.packets / cyc, "unit": "photons/s", "cores": threads, "kind": "port",
+            "sample": (f"FP64 oracle: {npk} packets of this tissue ({pk_rate:.4g} packets/s) and "
+                       f"{steps} full-grid heat steps ({vu_rate:.4g} voxel-updates/s) on {threads} "
+                       f"threads, combined into one cycle of {prob.packets:.3g} packets + "
+                       f"{prob.steps} steps (extrapolated)"),
+            "packets_per_s": pk_rate, "voxel_updates_per_s": vu_rate, "cycle_s": cyc}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="coupling cycles timed")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed cycles first")
+    ap.add_argument("--packets", type=float, default=1e8)
+    ap.add_argument("--heat-steps", type=int, default=10000)
+    ap.add_argument("--grid", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from tam_amd import configs
+    from tam_amd.cycle import CycleRunner
+
+    n = args.grid
+    prob = configs.config1(n=(n, n, n), packets=int(args.packets), steps=args.heat_steps,
+                           cycles=args.steps)
+    run = CycleRunner(prob, seed=2024)
+    if args.warmup > 0:
+        run.run(args.warmup)
+        run.reset()
+    wall = run.run(args.steps)
+    T = run.totals
+    p = prob.params
+    nvox = p.nx * p.ny * p.nz
+    cycles = args.steps
+    value = T["packets"] / wall
+    vu = T["voxel_updates"] / wall
+    mc_s, heat_s = T["mc_ms"] / 1e3, T["heat_ms"] / 1e3
+    step_ms = T["heat_ms"] / max(1, T["heat_launches"])
+    stencil = {"bound": "hbm", "achieved": STENCIL_BYTES * nvox / (step_ms * 1e-3) / 1e9,
+               "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    stencil["frac"] = stencil["achieved"] / stencil["peak"]
+    stencil["traffic"] = None
+    mc_ops = (MC_OPS_CROSS * T["crossings"] + MC_OPS_INTER * T["interactions"]) / mc_s
+    mc_roof = {"bound": "valu", "achieved": mc_ops / 1e12, "peak": LANE_OPS_PEAK / 1e12,
+               "unit": "Tlane-op/s", "frac": mc_ops / LANE_OPS_PEAK}
+    roofline = dict(stencil, kernel="heat_step_kernel")
+    line = {
+        "metric": "photons/s + FD voxel-steps/s; coupled cycle time, denoised vs plain @<=2% err",
+        "value": value, "unit": "photons/s", "n_gpus": 1, "steps": cycles, "warmup": args.warmup,
+        "ms_per_step": wall / cycles * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"three-layer skin {n}^3 over 2x2x1 cm, Gaussian w=0.15 cm, "
+                               f"{prob.packets:.3g} packets + {prob.steps} FP32 stencil steps "
+                               "(FP64 Arrhenius each step) + ablation/rebuild per cycle",
+                   "global_batch": int(prob.packets), "seq_len": prob.steps, "parallelism": "none",
+                   "grid": [n, n, n], "packets_per_cycle": int(prob.packets),
+                   "heat_steps_per_cycle": prob.steps, "tally": "f64", "damage": "f64"},
+        "voxel_updates_per_s": vu,
+        "cycle_time_s": wall / cycles,
+        "mc_packets_per_s_kernel": T["packets"] / mc_s,
+        "stencil_voxel_updates_per_s_kernel": T["voxel_updates"] / heat_s,
+        "time_split_s": {"mc": mc_s, "heat": heat_s, "rebuild": T["rebuild_ms"] / 1e3,
+                         "normalise": T["normalise_ms"] / 1e3, "wall": wall},
+        "mc_events": {"crossings_per_packet": T["crossings"] / T["packets"],
+                      "interactions_per_packet": T["interactions"] / T["packets"]},
+        "roofline": roofline,
+        "roofline_mc": mc_roof,
+        "cpu_baseline": None,
+    }
+    if mc_s > heat_s:
+        line["roofline_note"] = "transport kernel dominates; its VALU roofline is roofline_mc"
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds)
+    run.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -97,6 +97,11 @@ typedef struct {
     int32_t device;              /* HIP device ordinal */
     int32_t heat_fp64;           /* 1: FP64 temperatures in the reference's operation order
                                     (needed for the boiling clamp); 0: FP32 T - t_ref */
+    int32_t removed_inert;       /* 1: no conduction across faces of removed (ablated) voxels.
+                                    The reference gives them air properties (3dFD.f90:350-357),
+                                    whose diffusivity is ~140x tissue's: explicit steps at the
+                                    tissue's stability limit then diverge once two removed
+                                    voxels touch.  0 keeps the reference rule. */
 } tam_params;
 
 typedef struct {

@@ -607,6 +607,9 @@ int or_heat_steps(or_heat_t* h, int loops, const double* laser_on)
     double* tn = (double*)malloc(sizeof(double) * ntot);
     const double *kp = h->kappa, *de = h->density, *hc = h->heatcap;
     const double s = h->dscale;
+    /* removed voxel: an interior cell with rhokap == 0 (extension) */
+#define RM(cc, ii, jj, kk) (h->removed_inert && (ii) >= 1 && (ii) <= nx && (jj) >= 1 && (jj) <= ny && \
+                            (kk) >= 1 && (kk) <= nz && h->rhokap[(cc)] == 0.0)
     const double idz2 = 1.0 / (h->dz * h->dz), idy2 = 1.0 / (h->dy * h->dy),
                  idx2 = 1.0 / (h->dx * h->dx);
     int bad = 0;
@@ -628,6 +631,11 @@ int or_heat_steps(or_heat_t* h, int loops, const double* laser_on)
                     t0[c + pxy] = ((kk - 0.5 * h->h_top) * t0[c] + h->h_top * h->T_amb) / den;
                 }
         }
+        double flux = 0.0;
+        /* z-slabs in parallel, as the reference's ranks split z (3dFD.f90:45-72);
+         * every voxel's update is independent (Jacobi), so the result does not
+         * depend on the thread count */
+#pragma omp parallel for reduction(+ : flux) schedule(static)
         for (int k = 1; k <= nz; ++k)          /* 3dFD.f90:115-186 */
             for (int j = 1; j <= ny; ++j)
                 for (int i = 1; i <= nx; ++i) {
@@ -639,6 +647,8 @@ int or_heat_steps(or_heat_t* h, int loops, const double* laser_on)
                     hPH = .5 * (hc[c] + hc[c + pxy]); hMH = .5 * (hc[c] + hc[c - pxy]);
                     a = s * (kMH / (dMH * hMH)) * idz2;
                     d = s * (kPH / (dPH * hPH)) * idz2;
+                    if (RM(c, i, j, k) || RM(c - pxy, i, j, k - 1)) a = 0.0;
+                    if (RM(c, i, j, k) || RM(c + pxy, i, j, k + 1)) d = 0.0;
                     b = 0.5 * (a + d);
                     u_zz = a * t0[c - pxy] - 2.0 * b * t0[c] + d * t0[c + pxy];
                     kPH = 0.5 * (kp[c] + kp[c + px]); kMH = 0.5 * (kp[c] + kp[c - px]);
@@ -646,6 +656,8 @@ int or_heat_steps(or_heat_t* h, int loops, const double* laser_on)
                     hPH = 0.5 * (hc[c] + hc[c + px]); hMH = 0.5 * (hc[c] + hc[c - px]);
                     a = s * (kMH / (dMH * hMH)) * idy2;
                     d = s * (kPH / (dPH * hPH)) * idy2;
+                    if (RM(c, i, j, k) || RM(c - px, i, j - 1, k)) a = 0.0;
+                    if (RM(c, i, j, k) || RM(c + px, i, j + 1, k)) d = 0.0;
                     b = 0.5 * (a + d);
                     u_yy = a * t0[c - px] - 2.0 * b * t0[c] + d * t0[c + px];
                     kPH = .5 * (kp[c] + kp[c + 1]); kMH = .5 * (kp[c] + kp[c - 1]);
@@ -653,11 +665,13 @@ int or_heat_steps(or_heat_t* h, int loops, const double* laser_on)
                     hPH = .5 * (hc[c] + hc[c + 1]); hMH = .5 * (hc[c] + hc[c - 1]);
                     a = s * (kMH / (dMH * hMH)) * idx2;
                     d = s * (kPH / (dPH * hPH)) * idx2;
+                    if (RM(c, i, j, k) || RM(c - 1, i - 1, j, k)) a = 0.0;
+                    if (RM(c, i, j, k) || RM(c + 1, i + 1, j, k)) d = 0.0;
                     b = 0.5 * (a + d);
                     u_xx = a * t0[c - 1] - 2.0 * b * t0[c] + d * t0[c + 1];
 
                     /* boundary energy flow (extension): halo-neighbour terms */
-                    if (i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz) {
+                    if ((i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz) && !RM(c, i, j, k)) {
                         double e = 0.0, cc = hc[c] * h->massVoxel * h->delt;
                         if (k == 1) e += s * ((.5 * (kp[c] + kp[c - pxy])) / ((.5 * (de[c] + de[c - pxy])) * (.5 * (hc[c] + hc[c - pxy])))) * idz2 * (t0[c - pxy] - t0[c]);
                         if (k == nz) e += s * ((.5 * (kp[c] + kp[c + pxy])) / ((.5 * (de[c] + de[c + pxy])) * (.5 * (hc[c] + hc[c + pxy])))) * idz2 * (t0[c + pxy] - t0[c]);
@@ -665,7 +679,7 @@ int or_heat_steps(or_heat_t* h, int loops, const double* laser_on)
                         if (j == ny) e += s * ((.5 * (kp[c] + kp[c + px])) / ((.5 * (de[c] + de[c + px])) * (.5 * (hc[c] + hc[c + px])))) * idy2 * (t0[c + px] - t0[c]);
                         if (i == 1) e += s * ((.5 * (kp[c] + kp[c - 1])) / ((.5 * (de[c] + de[c - 1])) * (.5 * (hc[c] + hc[c - 1])))) * idx2 * (t0[c - 1] - t0[c]);
                         if (i == nx) e += s * ((.5 * (kp[c] + kp[c + 1])) / ((.5 * (de[c] + de[c + 1])) * (.5 * (hc[c] + hc[c + 1])))) * idx2 * (t0[c + 1] - t0[c]);
-                        h->flux_out -= cc * e;
+                        flux -= cc * e;
                     }
 
                     double tempIncrease = h->delt * (u_xx + u_yy + u_zz);
@@ -684,8 +698,10 @@ int or_heat_steps(or_heat_t* h, int loops, const double* laser_on)
                         tn[c] = tn[c] + tempIncrease + lon * h->coeff[c] * jt;
                     }
                 }
+        h->flux_out += flux;
         memcpy(t0, tn, sizeof(double) * ntot);  /* t0 = tn (3dFD.f90:187) */
     }
+#undef RM
     /* collate: interior z planes come back (3dFD.f90:218-219) */
     memcpy(h->temp + pxy, t0 + pxy, sizeof(double) * pxy * nz);
     free(t0); free(tn);

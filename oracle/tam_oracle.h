@@ -144,6 +144,8 @@ typedef struct {
     int     robin_top;              /* extension: convective top face */
     double  h_top, T_amb;           /* W/m^2/K, K */
     double  flux_out;               /* accumulates energy through the boundary, J (extension) */
+    const double* rhokap;           /* halo; with removed_inert, interior cells with rhokap == 0 */
+    int     removed_inert;          /* extension: no conduction across faces of removed voxels */
 } or_heat_t;
 
 /* `loops` explicit steps of heat_sim_3D (3dFD.f90:113-215, single rank).

@@ -53,7 +53,8 @@ class Heat(C.Structure):
                 ("heatcap", C.c_void_p), ("coeff", C.c_void_p), ("Q", C.c_void_p),
                 ("jmean", C.c_void_p), ("volumeVoxel", C.c_double), ("massVoxel", C.c_double),
                 ("QVapor", C.c_double), ("boil", C.c_int), ("robin_top", C.c_int),
-                ("h_top", C.c_double), ("T_amb", C.c_double), ("flux_out", C.c_double)]
+                ("h_top", C.c_double), ("T_amb", C.c_double), ("flux_out", C.c_double),
+                ("rhokap", C.c_void_p), ("removed_inert", C.c_int)]
 
 
 class Arrh(C.Structure):

@@ -123,6 +123,7 @@ hipError_t build_coeffs(tam_ctx* c)
         a.gx = c->gx; a.gy = c->gy; a.gz = c->gz;
         a.robin = c->p.robin_top; a.h_top = c->p.h_top; a.t_amb = c->p.t_amb; a.t_ref = c->p.t_ref;
         a.dz = c->p.dz_m; a.robin_coef = c->robin;
+        a.rhokap = c->rhokap; a.removed_inert = c->p.removed_inert;
         return launch_coeffs(a, c->stream);
     }
     Coeff64Args a{};
@@ -134,6 +135,7 @@ hipError_t build_coeffs(tam_ctx* c)
     a.idz2 = 1.0 / (c->p.dz_m * c->p.dz_m);
     a.kappa = c->kappa; a.density = c->density; a.heatcap = c->heatcap;
     a.ax = c->ax; a.ay = c->ay; a.az = c->az;
+    a.rhokap = c->rhokap; a.removed_inert = c->p.removed_inert;
     return launch_coeffs64(a, c->stream);
 }
 
@@ -361,6 +363,7 @@ int tam_set_optics(tam_ctx* c, const uint8_t* mat, const tam_material* table, in
     HIPCHK(c, hipMemcpyAsync(c->mua_frac, frac, sizeof(frac), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->rhokap, rhokap, sizeof(double) * (size_t)H, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->mut, mut.data(), sizeof(float) * (size_t)V, hipMemcpyHostToDevice, c->stream));
+    if (c->thermal_set) HIPCHK(c, build_coeffs(c));     // removed voxels follow rhokap
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->optics_set = true;
     return TAM_OK;

@@ -117,9 +117,14 @@ __global__ __launch_bounds__(256) void coeff64_kernel(Coeff64Args a)
         return a.dscale * (kb / (db * hb)) * ih2;
     };
     const bool jin = j >= 1 && j <= a.ny, kin = k >= 1 && k <= a.nz, iin = i >= 1 && i <= a.nx;
-    a.ax[c] = (i <= a.nx && jin && kin) ? face(1, a.idx2) : 0.0;
-    a.ay[c] = (j <= a.ny && iin && kin) ? face(a.px, a.idy2) : 0.0;
-    a.az[c] = (k <= a.nz && iin && jin) ? face(a.pxy, a.idz2) : 0.0;
+    auto removed = [&](long cc, bool interior) { return a.removed_inert && interior && a.rhokap[cc] == 0.0; };
+    const bool rc = removed(c, iin && jin && kin);
+    a.ax[c] = (i <= a.nx && jin && kin && !rc && !removed(c + 1, i + 1 <= a.nx && jin && kin))
+                  ? face(1, a.idx2) : 0.0;
+    a.ay[c] = (j <= a.ny && iin && kin && !rc && !removed(c + a.px, j + 1 <= a.ny && iin && kin))
+                  ? face(a.px, a.idy2) : 0.0;
+    a.az[c] = (k <= a.nz && iin && jin && !rc && !removed(c + a.pxy, k + 1 <= a.nz && iin && jin))
+                  ? face(a.pxy, a.idz2) : 0.0;
 }
 
 template <bool B, bool R, bool D, bool E>

@@ -125,9 +125,16 @@ __global__ __launch_bounds__(256) void coeff_kernel(CoeffArgs a)
         return (float)(a.delt * (a.dscale * (kb / (db * hb)) * ih2));
     };
     const bool jin = j >= 1 && j <= a.ny, kin = k >= 1 && k <= a.nz, iin = i >= 1 && i <= a.nx;
-    a.gx[c] = (i <= a.nx && jin && kin) ? face(1, a.idx2) : 0.f;
-    a.gy[c] = (j <= a.ny && iin && kin) ? face(a.px, a.idy2) : 0.f;
-    a.gz[c] = (k <= a.nz && iin && jin) ? face(a.pxy, a.idz2) : 0.f;
+    // a removed voxel is an interior cell whose rhokap is 0 (halo cells are
+    // empty too, but they carry the boundary condition)
+    auto removed = [&](long cc, bool interior) { return a.removed_inert && interior && a.rhokap[cc] == 0.0; };
+    const bool rc = removed(c, iin && jin && kin);
+    a.gx[c] = (i <= a.nx && jin && kin && !rc && !removed(c + 1, i + 1 <= a.nx && jin && kin))
+                  ? face(1, a.idx2) : 0.f;
+    a.gy[c] = (j <= a.ny && iin && kin && !rc && !removed(c + a.px, j + 1 <= a.ny && iin && kin))
+                  ? face(a.px, a.idy2) : 0.f;
+    a.gz[c] = (k <= a.nz && iin && jin && !rc && !removed(c + a.pxy, k + 1 <= a.nz && iin && jin))
+                  ? face(a.pxy, a.idz2) : 0.f;
     if (a.robin && k == a.nz && iin && jin) {          // convective top (extension)
         const double kk = K[c] / a.dz;
         const double den = kk + 0.5 * a.h_top;

@@ -94,6 +94,8 @@ struct Coeff64Args {
     double dscale, idx2, idy2, idz2;
     const double* kappa; const double* density; const double* heatcap;
     double* ax; double* ay; double* az;
+    const double* rhokap;
+    int removed_inert;
 };
 
 struct RebuildArgs {
@@ -119,6 +121,8 @@ struct CoeffArgs {
     float* gx; float* gy; float* gz;
     int robin; double h_top, t_amb, t_ref, dz;
     float2* robin_coef;
+    const double* rhokap;      // removed voxels: interior cells with rhokap == 0
+    int removed_inert;         // zero conductance across faces of removed voxels
 };
 
 struct tam_ctx {

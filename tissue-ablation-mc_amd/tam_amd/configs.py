@@ -203,6 +203,7 @@ def _config_params(n, ext_cm, steps_dt_frac=0.9):
     p.h_top = H_TOP
     p.t_amb = T_AMB
     p.thermal_mode = lib.THERMAL_TABLE
+    p.removed_inert = 1                              # ablated voxels: no conduction (stability)
     p.n_ext_zhi = 1.0                                # air above
     p.n_ext_zlo = 0.0
     return p

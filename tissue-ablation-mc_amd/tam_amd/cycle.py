@@ -1,0 +1,65 @@
+"""One MC-heat coupling cycle, as the body of mcpolar's time loop
+(mcpolar.f90:148-186): photon transport, normalisation, `loops` heat steps,
+Arrhenius damage (at the reference cadence unless it is integrated inside
+every step), and the ablation / property rebuild."""
+import time as _time
+
+from . import lib
+
+
+class CycleRunner:
+    def __init__(self, prob, seed=0):
+        self.prob = prob
+        self.seed = seed
+        self.grid = prob.grid()
+        self.time = 0.0
+        self.cycle = 0
+        self.totals = dict(packets=0.0, crossings=0.0, interactions=0.0, mc_ms=0.0, heat_ms=0.0,
+                           heat_launches=0, rebuild_ms=0.0, normalise_ms=0.0, arrhenius_ms=0.0,
+                           voxel_updates=0.0)
+
+    def reset(self):
+        """Restore the problem's initial tissue (optics, temperature, damage)."""
+        p = self.prob
+        self.grid.set_optics(p.mat, p.table, p.rhokap)
+        self.grid.set_thermal(p.temp, p.kappa, p.density, p.heatcap)
+        self.grid.upload(lib.FIELD_TISSUE, 0.0 * p.mat.astype(float))
+        self.grid.upload(lib.FIELD_THRESTIME, 0.0 * (p.mat.astype(float)[None].repeat(3, 0)))
+        self.time = 0.0
+        self.cycle = 0
+        for k in self.totals:
+            self.totals[k] = 0.0
+
+    def run_cycle(self):
+        p, g = self.prob, self.grid
+        st = g.mc_run(p.packets, seed=self.seed, stream=self.cycle)
+        g.normalise(p.scale)
+        g.heat_run(p.steps)
+        self.time += p.steps * p.params.delt
+        if not p.params.damage_every_step:
+            g.arrhenius(p.params.delt, self.time)
+        g.rebuild()
+        t = g.timing()
+        T = self.totals
+        T["packets"] += st.packets
+        T["crossings"] += st.crossings
+        T["interactions"] += st.interactions
+        T["mc_ms"] += st.kernel_ms
+        T["heat_ms"] += t.heat_ms
+        T["heat_launches"] += t.heat_steps
+        T["rebuild_ms"] += t.rebuild_ms
+        T["normalise_ms"] += t.normalise_ms
+        T["arrhenius_ms"] += 0.0 if p.params.damage_every_step else t.arrhenius_ms
+        T["voxel_updates"] += float(p.steps) * p.params.nx * p.params.ny * p.params.nz
+        self.cycle += 1
+        return st
+
+    def run(self, cycles):
+        t0 = _time.perf_counter()
+        for _ in range(cycles):
+            self.run_cycle()
+        self.grid.synchronize()
+        return _time.perf_counter() - t0
+
+    def close(self):
+        self.grid.close()

@@ -45,7 +45,7 @@ class Params(C.Structure):
         ("damage_every_step", C.c_int32),
         ("beam", C.c_int32), ("beam_radius", C.c_double),
         ("n_ext_zlo", C.c_double), ("n_ext_zhi", C.c_double),
-        ("device", C.c_int32), ("heat_fp64", C.c_int32),
+        ("device", C.c_int32), ("heat_fp64", C.c_int32), ("removed_inert", C.c_int32),
     ]
 
 
