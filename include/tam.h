@@ -122,6 +122,8 @@ typedef struct {
     double mc_ms, normalise_ms, heat_ms, arrhenius_ms, rebuild_ms;   /* last call of each */
     double heat_steps;           /* steps in the last tam_heat_run */
     double heat_step_ms;         /* mean device time of one stencil launch in the last tam_heat_run */
+    double class_mode;           /* 1: stencil runs on thermal classes + LDS coefficient table */
+    double tile, kz;             /* stencil launch shape in use */
 } tam_timing;
 
 /* --- lifetime -------------------------------------------------------- */
@@ -161,6 +163,16 @@ int tam_normalise(tam_ctx* ctx, double scale);
  * NULL: laser on) is advanced exactly as 3dFD.f90:199-214.  With
  * damage_every_step the Arrhenius integral runs inside each step. */
 int tam_heat_run(tam_ctx* ctx, int32_t loops, tam_pulse* pulse);
+
+/* --- stencil launch shape (no reference counterpart: the reference's z-slab
+ * split is fixed by the rank count, 3dFD.f90:45-72).  tile: 0 = 64x4,
+ * 1 = 32x8, 2 = 64x8 (x by y) voxels per block with x-y planes staged in
+ * LDS; 3, 4, 5 = the same tiles loading neighbours directly (no LDS, no
+ * barriers); kz: z planes per block.
+ * tam_tune_heat times every candidate on the current field (the result goes
+ * to the scratch buffer, the state is not advanced) and keeps the fastest. */
+int tam_set_heat_launch(tam_ctx* ctx, int32_t tile, int32_t kz);
+int tam_tune_heat(tam_ctx* ctx, int32_t reps, int32_t* tile, int32_t* kz, double* step_ms);
 
 /* --- Arrhenius (3dFD.f90:424-466), once, with the given delt and time. */
 int tam_arrhenius(tam_ctx* ctx, double delt, double time);

@@ -12,6 +12,10 @@
 
 #include "tam_internal.h"
 
+// Host-side coefficient tables restate the device's FP64 formulas; no
+// contraction so both round identically.
+#pragma clang fp contract(off)
+
 namespace {
 
 thread_local std::string g_create_err;
@@ -72,7 +76,91 @@ HeatArgs heat_args(tam_ctx* c)
     a.arr_Tlo = c->p.arr_Tlo; a.arr_Thi = c->p.arr_Thi;
     a.thr0 = c->p.arr_thr[0]; a.thr1 = c->p.arr_thr[1]; a.thr2 = c->p.arr_thr[2];
     a.energy = c->energy; a.flags = c->flags;
+    a.cls = c->cls; a.glut = c->glut; a.robin_lut = c->robin_lut;
+    a.eblk = c->eblk;
+    a.lnA = std::log(c->p.arr_A);
+    a.thr_min = std::fmin(c->p.arr_thr[0], std::fmin(c->p.arr_thr[1], c->p.arr_thr[2]));
+    a.dT_win_pad = 2.0f;
+    // FP32 pre-tests widened by a margin well above FP32 rounding; the exact
+    // comparisons are made in FP64 behind them
+    auto margin = [](double x) { return 1e-3 * std::fabs(x) + 1e-3; };
+    a.dT_neg = (float)(-c->p.t_ref);
+    const double db = 373.0 - c->p.t_ref, dl = c->p.arr_Tlo - c->p.t_ref, dh = c->p.arr_Thi - c->p.t_ref;
+    a.dT_boil_test = (float)(db - margin(db));
+    a.dT_lo_test = (float)(dl - margin(dl));
+    a.dT_hi_test = (float)(dh + margin(dh));
     return a;
+}
+
+// Thermal classes for the FP32 stencil's class mode: every distinct
+// (kappa, density, heatcap) triple over the halo grid gets an id (at most
+// TAM_NCLASS-1); removed interior voxels get TAM_CLASS_REMOVED.  The face
+// table holds the same FP64 expression as coeff_kernel, rounded to FP32.
+// Class mode needs the FP32 path, the table thermal model and inert removed
+// voxels (air properties depend on T, so they are not a fixed set).
+int classify(tam_ctx* c, const double* K, const double* D, const double* Hc, const double* rk)
+{
+    c->lut = false;
+    if (c->fp64 || c->p.thermal_mode != TAM_THERMAL_TABLE || !c->p.removed_inert) return TAM_OK;
+    struct Tri { double k, d, h; };
+    std::vector<Tri> u;
+    const long H = c->nhalo, px = c->px, pxy = c->pxy;
+    std::vector<uint8_t> cls((size_t)H);
+    for (long h = 0; h < H; ++h) {
+        const int i = (int)(h % px), j = (int)((h / px) % (c->p.ny + 2)), k = (int)(h / pxy);
+        const bool interior = i >= 1 && i <= c->p.nx && j >= 1 && j <= c->p.ny && k >= 1 && k <= c->p.nz;
+        if (interior && rk[h] == 0.0) {
+            cls[(size_t)h] = TAM_CLASS_REMOVED;
+            continue;
+        }
+        size_t m = 0;
+        while (m < u.size() && !(u[m].k == K[h] && u[m].d == D[h] && u[m].h == Hc[h])) ++m;
+        if (m == u.size()) {
+            if (u.size() == TAM_NCLASS - 1) return TAM_OK;      // too many: array mode
+            u.push_back(Tri{K[h], D[h], Hc[h]});
+        }
+        cls[(size_t)h] = (uint8_t)m;
+    }
+    std::vector<float> lut(3 * 256 + TAM_NCLASS, 0.f);       // + heatcap*mass per class
+    std::vector<float2> rob(TAM_NCLASS, make_float2(0.f, 0.f));
+    const double ih2[3] = {1.0 / (c->p.dx_m * c->p.dx_m), 1.0 / (c->p.dy_m * c->p.dy_m),
+                           1.0 / (c->p.dz_m * c->p.dz_m)};
+    for (int ax = 0; ax < 3; ++ax)
+        for (size_t m = 0; m < u.size(); ++m)
+            for (size_t n = 0; n < u.size(); ++n) {
+                const double kb = .5 * (u[m].k + u[n].k);
+                const double db = .5 * (u[m].d + u[n].d);
+                const double hb = .5 * (u[m].h + u[n].h);
+                lut[ax * 256 + m * 16 + n] = (float)(c->p.delt * (c->p.dscale * (kb / (db * hb)) * ih2[ax]));
+            }
+    for (size_t m = 0; m < u.size(); ++m) lut[3 * 256 + m] = (float)(u[m].h * c->p.mass_voxel);
+    for (size_t m = 0; m < u.size(); ++m) {
+        const double kk = u[m].k / c->p.dz_m;
+        const double den = kk + 0.5 * c->p.h_top;
+        const double beta = (kk - 0.5 * c->p.h_top) / den;
+        const double gam = c->p.h_top * c->p.t_amb / den;
+        rob[m] = make_float2((float)beta, (float)(beta * c->p.t_ref + gam - c->p.t_ref));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->cls, cls.data(), (size_t)H, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->glut, lut.data(), sizeof(float) * lut.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->robin_lut, rob.data(), sizeof(float2) * rob.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->lut = true;
+    return TAM_OK;
+}
+
+// classify() from the device copies of the properties (after optics change)
+int reclassify_from_device(tam_ctx* c)
+{
+    const size_t H = (size_t)c->nhalo;
+    std::vector<double> K(H), D(H), Hc(H), rk(H);
+    HIPCHK(c, hipMemcpyAsync(K.data(), c->kappa, 8 * H, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D.data(), c->density, 8 * H, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Hc.data(), c->heatcap, 8 * H, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(rk.data(), c->rhokap, 8 * H, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return classify(c, K.data(), D.data(), Hc.data(), rk.data());
 }
 
 RebuildArgs rebuild_args(tam_ctx* c)
@@ -89,6 +177,7 @@ RebuildArgs rebuild_args(tam_ctx* c)
     a.coeff = c->coeff; a.W = c->W; a.Q = c->Q; a.mut = c->mut; a.mat = c->mat;
     a.mask = c->mask; a.air_id = c->air_id;
     a.T64 = c->fp64 ? c->cur64 : nullptr;
+    a.cls = c->lut ? c->cls : nullptr;
     return a;
 }
 
@@ -233,6 +322,14 @@ int tam_create(const tam_params* p, tam_ctx** out)
     if (e == hipSuccess) e = dalloc(&c->mut, V);
     if (e == hipSuccess) e = dalloc(&c->mat, V);
     if (e == hipSuccess) e = dalloc(&c->mask, V);
+    if (e == hipSuccess) e = dalloc(&c->cls, H);
+    if (e == hipSuccess) e = dalloc(&c->glut, 3 * 256 + TAM_NCLASS);
+    if (e == hipSuccess) e = dalloc(&c->robin_lut, TAM_NCLASS);
+    c->tile = 1;
+    c->kz = 16;
+    // every stencil tile is at least 32 x 4 and kz >= 1
+    c->neblk = ((nx + 31) / 32) * ((ny + 3) / 4) * nz;
+    if (e == hipSuccess) e = dalloc(&c->eblk, c->neblk);
     if (e == hipSuccess) e = dalloc(&c->mtab, TAM_MAX_MATERIALS + 1);
     if (e == hipSuccess) e = dalloc(&c->mua_frac, TAM_MAX_MATERIALS + 1);
     if (e == hipSuccess) e = dalloc(&c->tally, V);
@@ -272,6 +369,7 @@ int tam_create(const tam_params* p, tam_ctx** out)
     launch_fill_f64(c->omega, V, 0.0, s);
     launch_fill_f64(c->thres, 3 * V, 0.0, s);
     launch_fill_f64(c->energy, 4, 0.0, s);
+    launch_fill_f64(c->eblk, c->neblk, 0.0, s);
     launch_fill_f32((float*)c->flags, 4, 0.f, s);
     launch_fill_f32(c->mut, V, 0.f, s);
     e = hipMemsetAsync(c->mat, 0, (size_t)V, s);
@@ -298,7 +396,8 @@ void tam_destroy(tam_ctx* c)
     void* ptrs[] = {c->dT0, c->dT1, c->gx, c->gy, c->gz, c->src, c->robin, c->kappa, c->density,
                     c->heatcap, c->coeff, c->rhokap, c->mut, c->mat, c->mask, c->mtab, c->mua_frac,
                     c->tally, c->jmean, c->Q, c->W, c->omega, c->thres, c->energy, c->flags,
-                    c->counters, c->T64a, c->T64b, c->ax, c->ay, c->az};
+                    c->counters, c->T64a, c->T64b, c->ax, c->ay, c->az, c->cls, c->glut,
+                    c->robin_lut, c->eblk};
     for (void* q : ptrs)
         if (q) hipFree(q);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -363,7 +462,11 @@ int tam_set_optics(tam_ctx* c, const uint8_t* mat, const tam_material* table, in
     HIPCHK(c, hipMemcpyAsync(c->mua_frac, frac, sizeof(frac), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->rhokap, rhokap, sizeof(double) * (size_t)H, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->mut, mut.data(), sizeof(float) * (size_t)V, hipMemcpyHostToDevice, c->stream));
-    if (c->thermal_set) HIPCHK(c, build_coeffs(c));     // removed voxels follow rhokap
+    if (c->thermal_set) {                               // removed voxels follow rhokap
+        HIPCHK(c, build_coeffs(c));
+        const int rc = reclassify_from_device(c);
+        if (rc != TAM_OK) return rc;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->optics_set = true;
     return TAM_OK;
@@ -405,6 +508,7 @@ int tam_set_thermal(tam_ctx* c, const double* temp, const double* kappa, const d
     if (W) HIPCHK(c, hipMemcpyAsync(c->W, W, sizeof(double) * (size_t)V, hipMemcpyHostToDevice, c->stream));
     else HIPCHK(c, launch_fill_f64(c->W, V, 0.75, c->stream));
     HIPCHK(c, launch_fill_f64(c->energy, 4, 0.0, c->stream));
+    HIPCHK(c, launch_fill_f64(c->eblk, c->neblk, 0.0, c->stream));
     HIPCHK(c, launch_fill_f32((float*)c->flags, 4, 0.f, c->stream));
     c->cur = c->dT0;
     c->nxt = c->dT1;
@@ -413,6 +517,13 @@ int tam_set_thermal(tam_ctx* c, const double* temp, const double* kappa, const d
     HIPCHK(c, build_coeffs(c));
     HIPCHK(c, refresh_src(c) == TAM_OK ? hipSuccess : hipErrorUnknown);
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->lut = false;
+    if (c->optics_set) {
+        std::vector<double> rk((size_t)H);
+        HIPCHK(c, hipMemcpy(rk.data(), c->rhokap, sizeof(double) * (size_t)H, hipMemcpyDeviceToHost));
+        const int rc = classify(c, kappa, density, heatcap, rk.data());
+        if (rc != TAM_OK) return rc;
+    }
     c->thermal_set = true;
     return TAM_OK;
 }
@@ -495,7 +606,8 @@ int tam_heat_run(tam_ctx* c, int32_t loops, tam_pulse* pulse)
             HIPCHK(c, launch_heat64_step(a64, c->cur64, c->nxt64, lon, t_after, boil, robin, dmg, 1, c->stream));
             std::swap(c->cur64, c->nxt64);
         } else {
-            HIPCHK(c, launch_heat_step(a, c->cur, c->nxt, (float)lon, t_after, boil, robin, dmg, 1, c->stream));
+            HIPCHK(c, launch_heat_step(a, c->cur, c->nxt, (float)lon, t_after, boil, robin, dmg, 1, c->lut ? 1 : 0,
+                                       c->tile, c->kz, c->stream));
             std::swap(c->cur, c->nxt);
         }
         if (pulse) {
@@ -641,6 +753,10 @@ int tam_upload(tam_ctx* c, int32_t field, const double* in)
         HIPCHK(c, hipMemcpyAsync(c->mut, mut.data(), sizeof(float) * (size_t)V, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->rhokap, in, sizeof(double) * (size_t)H, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->thermal_set) {                        // removed voxels follow rhokap
+            HIPCHK(c, build_coeffs(c));
+            return reclassify_from_device(c);
+        }
         return TAM_OK;
     }
     case TAM_FIELD_JMEAN: dst = c->jmean; n = V; break;
@@ -668,8 +784,11 @@ int tam_energy(tam_ctx* c, double* heat_content, double* boundary_out)
     HIPCHK(c, launch_energy(c->p.nx, c->p.ny, c->p.nz, c->px, c->pxy, c->cur, c->fp64 ? c->cur64 : nullptr,
                             c->p.t_ref, c->heatcap, c->p.mass_voxel, c->energy + 1, c->stream));
     double e[2];
+    std::vector<double> blk((size_t)c->neblk);
     HIPCHK(c, hipMemcpyAsync(e, c->energy, sizeof(e), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(blk.data(), c->eblk, sizeof(double) * blk.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (double b : blk) e[0] += b;
     if (boundary_out) *boundary_out = e[0];
     if (heat_content) *heat_content = e[1];
     return TAM_OK;
@@ -679,6 +798,54 @@ int tam_get_timing(tam_ctx* c, tam_timing* out)
 {
     if (!c || !out) return TAM_ERR_ARG;
     *out = c->timing;
+    out->class_mode = c->lut ? 1.0 : 0.0;
+    out->tile = c->tile;
+    out->kz = c->kz;
+    return TAM_OK;
+}
+
+int tam_set_heat_launch(tam_ctx* c, int32_t tile, int32_t kz)
+{
+    if (!c) return TAM_ERR_ARG;
+    if (tile < 0 || tile >= heat_num_tiles() || kz < 1) return fail(c, TAM_ERR_ARG, "bad stencil launch shape");
+    c->tile = tile;
+    c->kz = kz;
+    return TAM_OK;
+}
+
+int tam_tune_heat(tam_ctx* c, int32_t reps, int32_t* tile, int32_t* kz, double* step_ms)
+{
+    if (!c) return TAM_ERR_ARG;
+    if (!c->thermal_set) return fail(c, TAM_ERR_STATE, "tam_set_thermal must come first");
+    if (reps < 1) reps = 1;
+    Dev d(c->dev);
+    double best = 1e300;
+    int bt = c->tile, bk = c->kz;
+    if (!c->fp64) {
+        const HeatArgs a = heat_args(c);
+        const int kzs[] = {4, 8, 16, 32, 64};
+        for (int t = 0; t < heat_num_tiles(); ++t)
+            for (int kk : kzs) {
+                if (kk > c->p.nz && kk != 4) continue;
+                // cur -> nxt without swapping: the state is not advanced; no
+                // damage, boiling or energy side effects
+                HIPCHK(c, launch_heat_step(a, c->cur, c->nxt, 1.f, 0.0, 0, c->p.robin_top, 0, 0, c->lut ? 1 : 0, t,
+                                           kk, c->stream));
+                HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+                for (int r = 0; r < reps; ++r)
+                    HIPCHK(c, launch_heat_step(a, c->cur, c->nxt, 1.f, 0.0, 0, c->p.robin_top, 0, 0,
+                                               c->lut ? 1 : 0, t, kk, c->stream));
+                HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+                HIPCHK(c, hipEventSynchronize(c->ev1));
+                const double ms = elapsed(c) / reps;
+                if (ms < best) { best = ms; bt = t; bk = kk; }
+            }
+        c->tile = bt;
+        c->kz = bk;
+    }
+    if (tile) *tile = c->tile;
+    if (kz) *kz = c->kz;
+    if (step_ms) *step_ms = best < 1e300 ? best : 0.0;
     return TAM_OK;
 }
 

@@ -1,21 +1,27 @@
-// heat.hip -- explicit 7-point heat step and Arrhenius damage on CDNA4.
+// heat.hip -- FP32 explicit 7-point heat step and Arrhenius damage on CDNA4.
 //
 // Replaces the time loop of heat_sim_3D (3dFD.f90:113-215) and Arrhenius
-// (3dFD.f90:424-466).
+// (3dFD.f90:424-466) for the FP32 path (temperatures held as dT = T - t_ref).
 //
 // The reference's update, per axis, is  a*T(-) - 2b*T + d*T(+)  with
 // b = (a+d)/2 and half-node coefficients a, d built from arithmetic means of
-// kappa, density and heatcap (3dFD.f90:119-164).  That is
-// sum_faces G_f * (T_nbr - T) with G_f = delt * a_f, so the coefficients are
-// built once per property change (rebuild.hip) as FP32 face arrays gx/gy/gz,
-// and a step reads: dT (centre, 6 neighbours), 3 own face coefficients
-// (the -x/-y/-z ones are the neighbours' +faces), and the source.
+// kappa, density and heatcap (3dFD.f90:119-164), i.e.
+//     dT' = dT + sum_faces G_f * (dT_nbr - dT) + laserOn * src,   G_f = delt * a_f.
+// The face coefficients G only change when properties do (once per cycle),
+// so they are precomputed in one of two forms:
+//   * class mode: every halo cell carries a uint8 thermal class (the distinct
+//     (kappa, rho, c) triples, plus a "removed" class with no conduction) and
+//     G for each (axis, class, class) pair sits in a 3x16x16 LUT in LDS.  A
+//     voxel-update then moves 13 bytes (dT read + write, src, class).
+//   * array mode (any per-voxel properties): FP32 arrays gx/gy/gz hold each
+//     voxel's +x/+y/+z face; the -faces are the neighbours' +faces.
+// Both forms hold bit-identical G values (same FP64 formula, then rounded).
 //
-// Kernel shape: a block owns 256 consecutive (x,y) columns of the interior
-// (x fastest, so loads are coalesced along x) and marches KZ planes in z,
-// keeping T(k-1), T(k), T(k+1) and gz(k-1) in registers.  The +-x and +-y
-// neighbour loads hit the same or adjacent cache lines as the block's own
-// rows.  No LDS and no barrier: the exit of a lane never blocks others.
+// Kernel shape (2.5D blocking): a TX x TY block owns an x-y tile and marches
+// KZ planes in z.  Each plane of dT (and classes or gx/gy) is staged in LDS
+// with a one-cell halo so the +-x/+-y neighbours come from LDS; the +-z
+// neighbours stay in registers.  Loads for plane k+1 are issued before plane
+// k is computed, so they are in flight during the LDS exchange and the math.
 #include "tam_internal.h"
 
 namespace {
@@ -27,90 +33,366 @@ __device__ __forceinline__ double wave_sum_d(double v)
     return v;
 }
 
-template <bool BOIL, bool ROBIN, bool DAMAGE, bool ENERGY>
-__global__ __launch_bounds__(256) void heat_step_kernel(HeatArgs a, const float* __restrict__ Tin,
-                                                        float* __restrict__ Tout, float lon,
-                                                        double time_after, int kz)
+enum : unsigned { F_BOIL = 1u, F_ROBIN = 2u, F_DAMAGE = 4u, F_ENERGY = 8u };
+
+template <int TX, int TY, bool LUT>
+__global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const float* __restrict__ Tin,
+                                                         float* __restrict__ Tout, float lon, double t_after,
+                                                         int kz, unsigned flags)
 {
-    const int nxy = a.nx * a.ny;
-    const int lin = blockIdx.x * 256 + threadIdx.x;
-    double e_out = 0.0;
-    if (lin < nxy) {
-        const int i = lin % a.nx + 1;
-        const int j = lin / a.nx + 1;
-        const int k0 = blockIdx.y * kz + 1;
-        const int k1 = min(k0 + kz, a.nz + 1);
-        const long px = a.px, pxy = a.pxy;
-        long c = i + px * j + pxy * k0;
-        float tm = Tin[c - pxy];
-        float tc = Tin[c];
-        float gzm = a.gz[c - pxy];
-        const float tneg = -(float)a.t_ref;
-        for (int k = k0; k < k1; ++k, c += pxy) {
-            float tp;
-            if (ROBIN && k == a.nz) {
-                const float2 rb = a.robin[lin];
-                tp = rb.x * tc + rb.y;
+    __shared__ float sT[TY + 2][TX + 2];
+    __shared__ uint8_t sC[LUT ? TY + 2 : 1][LUT ? TX + 2 : 1];
+    __shared__ float sGx[LUT ? 1 : TY][LUT ? 1 : TX + 1];
+    __shared__ float sGy[LUT ? 1 : TY + 1][LUT ? 1 : TX];
+    __shared__ float sLut[LUT ? 3 * 256 + 16 : 1];   // face table + heatcap*mass per class
+    __shared__ float2 sRob[LUT ? 16 : 1];
+
+    const float* __restrict__ gxa = a.gx;
+    const float* __restrict__ gya = a.gy;
+    const float* __restrict__ gza = a.gz;
+    const float* __restrict__ srca = a.src;
+    const uint8_t* __restrict__ cla = a.cls;
+    const long px = a.px, pxy = a.pxy;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int i = blockIdx.x * TX + tx + 1, j = blockIdx.y * TY + ty + 1;
+    const bool in = i <= nx && j <= ny;
+    const bool bnd_ij = i == 1 || i == nx || j == 1 || j == ny;     // lane on an x or y face
+    // lanes past the grid edge load the halo cell (a valid address), never store
+    const int ic = i <= nx + 1 ? i : nx + 1, jc = j <= ny + 1 ? j : ny + 1;
+    const int k0 = blockIdx.z * kz + 1;
+    const int k1 = min(k0 + kz, nz + 1);
+    const bool robin = flags & F_ROBIN, damage = flags & F_DAMAGE, boil = flags & F_BOIL;
+    const bool energy = (flags & F_ENERGY) &&
+                        (blockIdx.x == 0 || blockIdx.x == gridDim.x - 1 || blockIdx.y == 0 ||
+                         blockIdx.y == gridDim.y - 1 || k0 == 1 || k1 == nz + 1);
+    // halo duties of this lane
+    const bool hxm = tx == 0, hxp = tx == TX - 1 && i <= nx, hym = ty == 0, hyp = ty == TY - 1 && j <= ny;
+
+    if (LUT) {
+        for (int t = threadIdx.x; t < 3 * 256 + 16; t += TX * TY) sLut[t] = a.glut[t];
+        if (threadIdx.x < 16) sRob[threadIdx.x] = a.robin_lut[threadIdx.x];
+    }
+
+    long c = ic + px * jc + pxy * k0;
+    // z registers: dT and class (or gz) of the planes below / at / above
+    float tm = Tin[c - pxy], tc = Tin[c], tp = Tin[c + pxy];
+    uint8_t cm = 0, cc = 0, cp = 0;
+    float gzm = 0.f, gzc = 0.f;
+    if (LUT) {
+        cm = cla[c - pxy]; cc = cla[c]; cp = cla[c + pxy];
+    } else {
+        gzm = gza[c - pxy]; gzc = gza[c];
+    }
+    // current-plane loads (x/y halo, coefficients, source).  Halo loads are
+    // guarded, never merged with a default: the value is only read by the
+    // same lane under the same guard, so no wait is forced on the load.
+    float hT0, hT1, hT2, hT3;
+    uint8_t hC0, hC1, hC2, hC3;
+    float gxc = 0.f, gyc = 0.f, hGx, hGy;
+    if (hxm) hT0 = Tin[c - 1];
+    if (hxp) hT1 = Tin[c + 1];
+    if (hym) hT2 = Tin[c - px];
+    if (hyp) hT3 = Tin[c + px];
+    if (LUT) {
+        if (hxm) hC0 = cla[c - 1];
+        if (hxp) hC1 = cla[c + 1];
+        if (hym) hC2 = cla[c - px];
+        if (hyp) hC3 = cla[c + px];
+    } else {
+        gxc = gxa[c]; gyc = gya[c];
+        if (hxm) hGx = gxa[c - 1];
+        if (hym) hGy = gya[c - px];
+    }
+    float s = srca[c];
+    float e_acc = 0.f;                          // boundary energy of this lane, J (per launch)
+    // damage integral: Omega is fetched one plane ahead for voxels whose
+    // temperature is already near the Arrhenius window (others load it late)
+    const long nxy = (long)nx * ny;
+    long v = (long)(ic - 1) + (long)nx * (jc - 1) + nxy * (k0 - 1);
+    const float wlo = a.dT_lo_test - a.dT_win_pad, whi = a.dT_hi_test + a.dT_win_pad;
+    bool pom = damage && in && tc >= wlo && tc < whi;
+    double om_pf = 0.0;
+    if (pom) om_pf = a.omega[v];
+
+    for (int k = k0; k < k1; ++k, c += pxy, v += nxy) {
+        __syncthreads();                       // previous plane's LDS reads are done
+        sT[ty + 1][tx + 1] = tc;
+        if (hxm) sT[ty + 1][0] = hT0;
+        if (hxp) sT[ty + 1][TX + 1] = hT1;
+        if (hym) sT[0][tx + 1] = hT2;
+        if (hyp) sT[TY + 1][tx + 1] = hT3;
+        if (LUT) {
+            sC[ty + 1][tx + 1] = cc;
+            if (hxm) sC[ty + 1][0] = hC0;
+            if (hxp) sC[ty + 1][TX + 1] = hC1;
+            if (hym) sC[0][tx + 1] = hC2;
+            if (hyp) sC[TY + 1][tx + 1] = hC3;
+        } else {
+            sGx[ty][tx + 1] = gxc;
+            if (hxm) sGx[ty][0] = hGx;
+            sGy[ty + 1][tx] = gyc;
+            if (hym) sGy[0][tx] = hGy;
+        }
+        const float tc_k = tc, tm_k = tm, tp_k0 = tp, s_k = s;
+        const int cc_k = cc, cm_k = cm, cp_k = cp;
+        const float gzm_k = gzm, gzc_k = gzc;
+        const bool pom_k = pom;
+        const double om_k = om_pf;
+        // ---- prefetch plane k+1 (addresses stay inside the halo array: k+2 <= nz+1
+        //      whenever k+1 < k1 <= nz+1; the last iteration skips it)
+        if (k + 1 < k1) {
+            const long n = c + pxy;
+            tm = tc; tc = tp; tp = Tin[n + pxy];
+            if (hxm) hT0 = Tin[n - 1];
+            if (hxp) hT1 = Tin[n + 1];
+            if (hym) hT2 = Tin[n - px];
+            if (hyp) hT3 = Tin[n + px];
+            if (LUT) {
+                cm = cc; cc = cp; cp = cla[n + pxy];
+                if (hxm) hC0 = cla[n - 1];
+                if (hxp) hC1 = cla[n + 1];
+                if (hym) hC2 = cla[n - px];
+                if (hyp) hC3 = cla[n + px];
             } else {
-                tp = Tin[c + pxy];
+                gzm = gzc; gzc = gza[n];
+                gxc = gxa[n]; gyc = gya[n];
+                if (hxm) hGx = gxa[n - 1];
+                if (hym) hGy = gya[n - px];
             }
-            const float gzp = a.gz[c];
-            const float gxm = a.gx[c - 1], gxp = a.gx[c];
-            const float gym = a.gy[c - px], gyp = a.gy[c];
-            const float txm = Tin[c - 1], txp = Tin[c + 1];
-            const float tym = Tin[c - px], typ = Tin[c + px];
-            const float inc = gxm * (txm - tc) + gxp * (txp - tc) + gym * (tym - tc) +
-                              gyp * (typ - tc) + gzm * (tm - tc) + gzp * (tp - tc);
-            float tn = tc + inc + lon * a.src[c];
-            if (tc < tneg) a.flags[0] = 1;              // 3dFD.f90:179-182
-            if (BOIL) {                                  // 3dFD.f90:167-172
+            s = srca[n];
+            pom = damage && in && tc >= wlo && tc < whi;      // tc now holds plane k+1
+            if (pom) om_pf = a.omega[v + nxy];
+        }
+        __syncthreads();
+        if (!in) continue;
+        // ---- update of voxel (i, j, k)
+        const float txm = sT[ty + 1][tx], txp = sT[ty + 1][tx + 2];
+        const float tym = sT[ty][tx + 1], typ = sT[ty + 2][tx + 1];
+        float gxm, gxp, gym, gyp, gzm_f, gzp_f;
+        if (LUT) {
+            const int cxm = sC[ty + 1][tx], cxp = sC[ty + 1][tx + 2];
+            const int cym = sC[ty][tx + 1], cyp = sC[ty + 2][tx + 1];
+            gxm = sLut[cxm * 16 + cc_k];
+            gxp = sLut[cc_k * 16 + cxp];
+            gym = sLut[256 + cym * 16 + cc_k];
+            gyp = sLut[256 + cc_k * 16 + cyp];
+            gzm_f = sLut[512 + cm_k * 16 + cc_k];
+            gzp_f = sLut[512 + cc_k * 16 + cp_k];
+        } else {
+            gxm = sGx[ty][tx]; gxp = sGx[ty][tx + 1];
+            gym = sGy[ty][tx]; gyp = sGy[ty + 1][tx];
+            gzm_f = gzm_k; gzp_f = gzc_k;
+        }
+        float tp_k = tp_k0;
+        if (robin && k == nz) {                 // ghost node of the convective top face
+            float2 rb;
+            if (LUT) rb = sRob[cc_k];
+            else rb = a.robin[(i - 1) + (long)nx * (j - 1)];
+            tp_k = rb.x * tc_k + rb.y;
+        }
+        const float inc = gxm * (txm - tc_k) + gxp * (txp - tc_k) + gym * (tym - tc_k) +
+                          gyp * (typ - tc_k) + gzm_f * (tm_k - tc_k) + gzp_f * (tp_k - tc_k);
+        float tn = tc_k + inc + lon * s_k;
+        if (tc_k < a.dT_neg) a.flags[0] = 1;                    // 3dFD.f90:179-182
+        if (boil && tc_k >= a.dT_boil_test) {                    // 3dFD.f90:167-172
+            const double T64 = a.t_ref + (double)tc_k;
+            const long v = (i - 1) + (long)nx * ((j - 1) + (long)ny * (k - 1));
+            const double q = a.Q[v];
+            if (T64 >= 100.0 + 273.0 && q < a.qvapor) {
+                const double e = (double)lon * a.jmean[v] * a.delt * a.volume_voxel +
+                                 a.heatcap[c] * a.mass_voxel * (double)inc;
+                if (e > 0.0) {
+                    a.Q[v] = fmin(q + e, a.qvapor);
+                    tn = a.dT_boil;
+                }
+            }
+        }
+        Tout[c] = tn;
+        if (damage && tn >= a.dT_lo_test && tn < a.dT_hi_test) {   // 3dFD.f90:450-461, every step
+            const double T = a.t_ref + (double)tn;
+            if (T >= a.arr_Tlo && T < a.arr_Thi) {
+                const long nv = nxy * nz;
+                // rate A exp(-dE/RT) = exp(ln A - dE/RT): exponent formed in
+                // FP64 (both terms ~230), exponentiated in FP32 (|arg| < ~40,
+                // <= 2e-6 relative), accumulated in FP64
+                const float arg = (float)(a.lnA - a.arr_dE / (a.arr_R * T));
+                const double om = (pom_k ? om_k : a.omega[v]) + a.delt * (double)expf(arg);
+                a.omega[v] = om;
+                // the threshold chain can only fire once Omega reaches the lowest threshold
+                if (om >= a.thr_min) {
+                    if (a.thres[v] == 0.0 && om >= a.thr0) a.thres[v] = t_after;
+                    else if (a.thres[v + nv] == 0.0 && om >= a.thr1) a.thres[v + nv] = t_after;
+                    else if (a.thres[v + 2 * nv] == 0.0 && om >= a.thr2) a.thres[v + 2 * nv] = t_after;
+                }
+            }
+        }
+        if (energy && (bnd_ij || k == 1 || k == nz)) {
+            float e = 0.f;
+            if (i == 1) e += gxm * (txm - tc_k);
+            if (i == nx) e += gxp * (txp - tc_k);
+            if (j == 1) e += gym * (tym - tc_k);
+            if (j == ny) e += gyp * (typ - tc_k);
+            if (k == 1) e += gzm_f * (tm_k - tc_k);
+            if (k == nz) e += gzp_f * (tp_k - tc_k);
+            e_acc -= (LUT ? sLut[768 + cc_k] : (float)(a.heatcap[c] * a.mass_voxel)) * e;
+        }
+    }
+    if (energy) {
+        // one FP64 slot per block, accumulated across steps (no shared-address
+        // atomics); tam_energy sums the slots
+        __shared__ double sE[TX * TY / 64];
+        const double w = wave_sum_d((double)e_acc);
+        if ((threadIdx.x & 63) == 0) sE[threadIdx.x >> 6] = w;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int q = 0; q < TX * TY / 64; ++q) t += sE[q];
+            const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+            a.eblk[b] += t;
+        }
+    }
+}
+
+// Direct form of the same update: no LDS and no barriers.  Every lane loads
+// its own +-x/+-y neighbours (they share cache lines with the tile's own
+// rows, so they are L1/L2 hits) from addresses clamped into the interior, so
+// the plane loop is straight-line code; +-z stay in registers.
+template <int TX, int TY, bool LUT>
+__global__ __launch_bounds__(TX* TY) void stencil_direct_kernel(const HeatArgs a, const float* __restrict__ Tin,
+                                                                float* __restrict__ Tout, float lon,
+                                                                double t_after, int kz, unsigned flags)
+{
+    __shared__ float sLut[LUT ? 3 * 256 + 16 : 1];   // face table + heatcap*mass per class
+    __shared__ float2 sRob[LUT ? 16 : 1];
+    const float* __restrict__ gxa = a.gx;
+    const float* __restrict__ gya = a.gy;
+    const float* __restrict__ gza = a.gz;
+    const float* __restrict__ srca = a.src;
+    const uint8_t* __restrict__ cla = a.cls;
+    const long px = a.px, pxy = a.pxy;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int i = blockIdx.x * TX + tx + 1, j = blockIdx.y * TY + ty + 1;
+    const bool in = i <= nx && j <= ny;
+    const bool bnd_ij = i == 1 || i == nx || j == 1 || j == ny;     // lane on an x or y face
+    const int ic = i <= nx ? i : nx, jc = j <= ny ? j : ny;     // clamped: every neighbour is valid
+    const int k0 = blockIdx.z * kz + 1;
+    const int k1 = min(k0 + kz, nz + 1);
+    const bool robin = flags & F_ROBIN, damage = flags & F_DAMAGE, boil = flags & F_BOIL;
+    const bool energy = (flags & F_ENERGY) &&
+                        (blockIdx.x == 0 || blockIdx.x == gridDim.x - 1 || blockIdx.y == 0 ||
+                         blockIdx.y == gridDim.y - 1 || k0 == 1 || k1 == nz + 1);
+    if (LUT) {
+        for (int t = threadIdx.x; t < 3 * 256 + 16; t += TX * TY) sLut[t] = a.glut[t];
+        if (threadIdx.x < 16) sRob[threadIdx.x] = a.robin_lut[threadIdx.x];
+        __syncthreads();
+    }
+    long c = ic + px * jc + pxy * k0;
+    const long nxy = (long)nx * ny;
+    long v = (long)(ic - 1) + (long)nx * (jc - 1) + nxy * (k0 - 1);
+    float tm = Tin[c - pxy], tc = Tin[c];
+    int cm = 0, cc = 0;
+    float gzm = 0.f;
+    if (LUT) {
+        cm = cla[c - pxy]; cc = cla[c];
+    } else {
+        gzm = gza[c - pxy];
+    }
+    float e_acc = 0.f;
+    for (int k = k0; k < k1; ++k, c += pxy, v += nxy) {
+        const float tp0 = Tin[c + pxy];
+        const float txm = Tin[c - 1], txp = Tin[c + 1], tym = Tin[c - px], typ = Tin[c + px];
+        const float s_k = srca[c];
+        // Omega issued with the plane's loads when T is already near the window
+        const bool pom = damage && in && tc >= a.dT_lo_test - a.dT_win_pad && tc < a.dT_hi_test + a.dT_win_pad;
+        double om_pf = 0.0;
+        if (pom) om_pf = a.omega[v];
+        float gxm, gxp, gym, gyp, gzp;
+        int cp = 0;
+        if (LUT) {
+            cp = cla[c + pxy];
+            const int cxm = cla[c - 1], cxp = cla[c + 1], cym = cla[c - px], cyp = cla[c + px];
+            gxm = sLut[cxm * 16 + cc];
+            gxp = sLut[cc * 16 + cxp];
+            gym = sLut[256 + cym * 16 + cc];
+            gyp = sLut[256 + cc * 16 + cyp];
+            gzp = sLut[512 + cc * 16 + cp];
+            gzm = sLut[512 + cm * 16 + cc];
+        } else {
+            gxm = gxa[c - 1]; gxp = gxa[c]; gym = gya[c - px]; gyp = gya[c]; gzp = gza[c];
+        }
+        float tp = tp0;
+        if (robin && k == nz) {
+            float2 rb;
+            if (LUT) rb = sRob[cc];
+            else rb = a.robin[(ic - 1) + (long)nx * (jc - 1)];
+            tp = rb.x * tc + rb.y;
+        }
+        const float inc = gxm * (txm - tc) + gxp * (txp - tc) + gym * (tym - tc) + gyp * (typ - tc) +
+                          gzm * (tm - tc) + gzp * (tp - tc);
+        float tn = tc + inc + lon * s_k;
+        if (in) {
+            if (tc < a.dT_neg) a.flags[0] = 1;                   // 3dFD.f90:179-182
+            if (boil && tc >= a.dT_boil_test) {                   // 3dFD.f90:167-172
                 const double T64 = a.t_ref + (double)tc;
-                if (T64 >= 100.0 + 273.0) {
-                    const long v = (i - 1) + (long)a.nx * ((j - 1) + (long)a.ny * (k - 1));
-                    const double q = a.Q[v];
-                    if (q < a.qvapor) {
-                        const double e = (double)lon * a.jmean[v] * a.delt * a.volume_voxel +
-                                         a.heatcap[c] * a.mass_voxel * (double)inc;
-                        if (e > 0.0) {
-                            a.Q[v] = fmin(q + e, a.qvapor);
-                            tn = a.dT_boil;
-                        }
+                const double q = a.Q[v];
+                if (T64 >= 100.0 + 273.0 && q < a.qvapor) {
+                    const double e = (double)lon * a.jmean[v] * a.delt * a.volume_voxel +
+                                     a.heatcap[c] * a.mass_voxel * (double)inc;
+                    if (e > 0.0) {
+                        a.Q[v] = fmin(q + e, a.qvapor);
+                        tn = a.dT_boil;
                     }
                 }
             }
             Tout[c] = tn;
-            if (DAMAGE) {                                // 3dFD.f90:450-461, every step
+            if (damage && tn >= a.dT_lo_test && tn < a.dT_hi_test) {  // 3dFD.f90:450-461
                 const double T = a.t_ref + (double)tn;
                 if (T >= a.arr_Tlo && T < a.arr_Thi) {
-                    const long v = (i - 1) + (long)a.nx * ((j - 1) + (long)a.ny * (k - 1));
-                    const long nv = (long)nxy * a.nz;
-                    const double om = a.omega[v] + a.delt * a.arr_A * exp(-a.arr_dE / (a.arr_R * T));
+                    const long nv = nxy * nz;
+                    const float arg = (float)(a.lnA - a.arr_dE / (a.arr_R * T));
+                    const double om = (pom ? om_pf : a.omega[v]) + a.delt * (double)expf(arg);
                     a.omega[v] = om;
-                    if (a.thres[v] == 0.0 && om >= a.thr0) a.thres[v] = time_after;
-                    else if (a.thres[v + nv] == 0.0 && om >= a.thr1) a.thres[v + nv] = time_after;
-                    else if (a.thres[v + 2 * nv] == 0.0 && om >= a.thr2) a.thres[v + 2 * nv] = time_after;
+                    if (om >= a.thr_min) {
+                        if (a.thres[v] == 0.0 && om >= a.thr0) a.thres[v] = t_after;
+                        else if (a.thres[v + nv] == 0.0 && om >= a.thr1) a.thres[v + nv] = t_after;
+                        else if (a.thres[v + 2 * nv] == 0.0 && om >= a.thr2) a.thres[v + 2 * nv] = t_after;
+                    }
                 }
             }
-            if (ENERGY) {
-                if (i == 1 || i == a.nx || j == 1 || j == a.ny || k == 1 || k == a.nz) {
-                    float e = 0.f;
-                    if (i == 1) e += gxm * (txm - tc);
-                    if (i == a.nx) e += gxp * (txp - tc);
-                    if (j == 1) e += gym * (tym - tc);
-                    if (j == a.ny) e += gyp * (typ - tc);
-                    if (k == 1) e += gzm * (tm - tc);
-                    if (k == a.nz) e += gzp * (tp - tc);
-                    e_out -= a.heatcap[c] * a.mass_voxel * (double)e;
-                }
+            if (energy && (bnd_ij || k == 1 || k == nz)) {
+                float e = 0.f;
+                if (i == 1) e += gxm * (txm - tc);
+                if (i == nx) e += gxp * (txp - tc);
+                if (j == 1) e += gym * (tym - tc);
+                if (j == ny) e += gyp * (typ - tc);
+                if (k == 1) e += gzm * (tm - tc);
+                if (k == nz) e += gzp * (tp - tc);
+                e_acc -= (LUT ? sLut[768 + cc] : (float)(a.heatcap[c] * a.mass_voxel)) * e;
             }
-            tm = tc;
-            tc = tp;
+        }
+        tm = tc;
+        tc = tp0;
+        if (LUT) {
+            cm = cc;
+            cc = cp;
+        } else {
             gzm = gzp;
         }
     }
-    if (ENERGY) {
-        const double w = wave_sum_d(e_out);
-        if ((threadIdx.x & 63) == 0 && w != 0.0) unsafeAtomicAdd(&a.energy[0], w);
+    if (energy) {
+        __shared__ double sE[TX * TY / 64];
+        const double w = wave_sum_d((double)e_acc);
+        if ((threadIdx.x & 63) == 0) sE[threadIdx.x >> 6] = w;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int q = 0; q < TX * TY / 64; ++q) t += sE[q];
+            const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+            a.eblk[b] += t;
+        }
     }
 }
 
@@ -139,37 +421,51 @@ __global__ __launch_bounds__(256) void arrhenius_kernel(HeatArgs a, const float*
     else if (a.thres[v + 2 * nv] == 0.0 && om >= a.thr2) a.thres[v + 2 * nv] = time;
 }
 
-template <bool B, bool R, bool D, bool E>
-hipError_t launch_t(const HeatArgs& a, const float* Tin, float* Tout, float lon, double t, hipStream_t s)
+template <int TX, int TY, bool LUT, bool DIRECT>
+hipError_t launch_t(const HeatArgs& a, const float* Tin, float* Tout, float lon, double t, int kz,
+                    unsigned flags, hipStream_t s)
 {
-    const int nxy = a.nx * a.ny;
-    const int bx = (nxy + 255) / 256;
-    // z chunk: enough blocks to fill 256 CUs several times over
-    int kz = (int)(((long)a.nz * bx) / 4096);
-    if (kz < 2) kz = 2;
-    if (kz > 32) kz = 32;
-    if (kz > a.nz) kz = a.nz;
-    const int bz = (a.nz + kz - 1) / kz;
-    hipLaunchKernelGGL((heat_step_kernel<B, R, D, E>), dim3(bx, bz), dim3(256), 0, s, a, Tin, Tout, lon, t, kz);
+    const dim3 grid((a.nx + TX - 1) / TX, (a.ny + TY - 1) / TY, (a.nz + kz - 1) / kz);
+    if (DIRECT)
+        hipLaunchKernelGGL((stencil_direct_kernel<TX, TY, LUT>), grid, dim3(TX * TY), 0, s, a, Tin, Tout, lon, t,
+                           kz, flags);
+    else
+        hipLaunchKernelGGL((stencil_kernel<TX, TY, LUT>), grid, dim3(TX * TY), 0, s, a, Tin, Tout, lon, t, kz,
+                           flags);
     return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_heat_step(const HeatArgs& a, const float* Tin, float* Tout, float lon,
-                            double time_after, int boil, int robin, int damage, int energy,
+// launch shapes: 0-2 stage x-y planes in LDS, 3-5 load neighbours directly
+int heat_num_tiles() { return 6; }
+const char* heat_tile_name(int t)
+{
+    static const char* names[] = {"lds-64x4", "lds-32x8", "lds-64x8", "direct-64x4", "direct-32x8", "direct-64x8"};
+    return (t >= 0 && t < 6) ? names[t] : "?";
+}
+
+hipError_t launch_heat_step(const HeatArgs& a, const float* Tin, float* Tout, float lon, double time_after,
+                            int boil, int robin, int damage, int energy, int lut, int tile, int kz,
                             hipStream_t s)
 {
-    const int key = (boil ? 8 : 0) | (robin ? 4 : 0) | (damage ? 2 : 0) | (energy ? 1 : 0);
-    switch (key) {
-#define TAM_CASE(K)                                                                        \
-    case K:                                                                                \
-        return launch_t<((K) & 8) != 0, ((K) & 4) != 0, ((K) & 2) != 0, ((K) & 1) != 0>( \
-            a, Tin, Tout, lon, time_after, s);
-        TAM_CASE(0) TAM_CASE(1) TAM_CASE(2) TAM_CASE(3) TAM_CASE(4) TAM_CASE(5) TAM_CASE(6)
-        TAM_CASE(7) TAM_CASE(8) TAM_CASE(9) TAM_CASE(10) TAM_CASE(11) TAM_CASE(12) TAM_CASE(13)
-        TAM_CASE(14) TAM_CASE(15)
-#undef TAM_CASE
+    const unsigned f = (boil ? F_BOIL : 0u) | (robin ? F_ROBIN : 0u) | (damage ? F_DAMAGE : 0u) |
+                       (energy ? F_ENERGY : 0u);
+    if (kz < 1) kz = 1;
+    if (kz > a.nz) kz = a.nz;
+    switch (tile * 2 + (lut ? 1 : 0)) {
+    case 0: return launch_t<64, 4, false, false>(a, Tin, Tout, lon, time_after, kz, f, s);
+    case 1: return launch_t<64, 4, true, false>(a, Tin, Tout, lon, time_after, kz, f, s);
+    case 2: return launch_t<32, 8, false, false>(a, Tin, Tout, lon, time_after, kz, f, s);
+    case 3: return launch_t<32, 8, true, false>(a, Tin, Tout, lon, time_after, kz, f, s);
+    case 4: return launch_t<64, 8, false, false>(a, Tin, Tout, lon, time_after, kz, f, s);
+    case 5: return launch_t<64, 8, true, false>(a, Tin, Tout, lon, time_after, kz, f, s);
+    case 6: return launch_t<64, 4, false, true>(a, Tin, Tout, lon, time_after, kz, f, s);
+    case 7: return launch_t<64, 4, true, true>(a, Tin, Tout, lon, time_after, kz, f, s);
+    case 8: return launch_t<32, 8, false, true>(a, Tin, Tout, lon, time_after, kz, f, s);
+    case 9: return launch_t<32, 8, true, true>(a, Tin, Tout, lon, time_after, kz, f, s);
+    case 10: return launch_t<64, 8, false, true>(a, Tin, Tout, lon, time_after, kz, f, s);
+    case 11: return launch_t<64, 8, true, true>(a, Tin, Tout, lon, time_after, kz, f, s);
     }
     return hipErrorInvalidValue;
 }

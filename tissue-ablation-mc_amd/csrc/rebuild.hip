@@ -90,6 +90,7 @@ __global__ __launch_bounds__(256) void props_kernel(RebuildArgs a)   // 3dFD.f90
         a.coeff[c] = a.delt / (air_density(T) * hc);
         a.mut[v] = 0.f;
         a.mat[v] = (uint8_t)a.air_id;
+        if (a.cls) a.cls[c] = (uint8_t)TAM_CLASS_REMOVED;
     } else if (a.mode == TAM_THERMAL_WATER) {
         const double W = a.W[v];
         const double rho = skin_density(W);

@@ -64,10 +64,23 @@ struct HeatArgs {
     double* thres;             // interior x3
     const double* rhokap;      // halo
     double arr_A, arr_dE, arr_R, arr_Tlo, arr_Thi, thr0, thr1, thr2;
+    double lnA;                // log(arr_A), for the in-step damage rate
+    double thr_min;            // smallest threshold of the chain
+    float dT_win_pad;          // extra margin of the Omega prefetch predicate, K
     // energy accounting
     double* energy;            // [0] boundary outflow, J
+    double* eblk;              // per-block boundary outflow slots (FP32 stencil)
     int* flags;                // [0] negative temperature seen
+    // class mode (see heat.hip)
+    const uint8_t* cls;        // halo: thermal class per cell
+    const float* glut;         // [3][16][16] face coefficient per (axis, minus class, plus class)
+    const float2* robin_lut;   // [16] convective ghost (beta, gamma') per top-cell class
+    // FP32 pre-tests (conservative) before the exact FP64 tests
+    float dT_neg, dT_boil_test, dT_lo_test, dT_hi_test;
 };
+
+#define TAM_NCLASS 16          // LUT classes; the last one is "removed" (no conduction)
+#define TAM_CLASS_REMOVED (TAM_NCLASS - 1)
 
 struct Heat64Args {            // FP64 reference-order step (heat64.hip)
     int nx, ny, nz;
@@ -111,6 +124,7 @@ struct RebuildArgs {
     float* mut; uint8_t* mat;
     uint8_t* mask;             // interior scratch
     int air_id;                // material id given to removed voxels
+    uint8_t* cls;              // halo thermal classes (class mode), nullptr otherwise
 };
 
 struct CoeffArgs {
@@ -139,6 +153,13 @@ struct tam_ctx {
     float *cur, *nxt;           // current / next temperature buffer (ping-pong)
     bool fp64;                  // FP64 reference-order heat path
     double *T64a, *T64b, *cur64, *nxt64, *ax, *ay, *az;
+    bool lut;                   // class mode active (FP32 path)
+    uint8_t* cls;               // halo classes
+    float* glut;                // [3][16][16]
+    float2* robin_lut;          // [16]
+    int tile, kz;               // stencil launch shape (autotuned)
+    double* eblk;               // per-block energy slots
+    long neblk;                 // slots: enough for the largest launch grid
     float2* robin;
     double *kappa, *density, *heatcap, *coeff, *rhokap;
     float* mut;
@@ -156,9 +177,11 @@ struct tam_ctx {
 
 // ---- launchers (each enqueues on `s`, returns hipError_t) --------------
 hipError_t launch_mc(const McArgs& a, hipStream_t s);
-hipError_t launch_heat_step(const HeatArgs& a, const float* Tin, float* Tout, float lon,
-                            double time_after, int boil, int robin, int damage, int energy,
+hipError_t launch_heat_step(const HeatArgs& a, const float* Tin, float* Tout, float lon, double time_after,
+                            int boil, int robin, int damage, int energy, int lut, int tile, int kz,
                             hipStream_t s);
+int heat_num_tiles();
+const char* heat_tile_name(int tile);
 hipError_t launch_heat64_step(const Heat64Args& a, const double* Tin, double* Tout, double lon,
                               double time_after, int boil, int robin, int damage, int energy, hipStream_t s);
 hipError_t launch_coeffs64(const Coeff64Args& a, hipStream_t s);

@@ -26,7 +26,7 @@ HALO_FIELDS = {FIELD_TEMP, FIELD_RHOKAP, FIELD_KAPPA, FIELD_DENSITY, FIELD_HEATC
 EXPORTED = ("tam_abi_version", "tam_create", "tam_destroy", "tam_last_error", "tam_set_optics",
             "tam_set_thermal", "tam_mc_run", "tam_normalise", "tam_heat_run", "tam_arrhenius",
             "tam_rebuild", "tam_download", "tam_upload", "tam_energy", "tam_get_timing",
-            "tam_synchronize")
+            "tam_synchronize", "tam_set_heat_launch", "tam_tune_heat")
 
 
 class Params(C.Structure):
@@ -77,7 +77,8 @@ class Pulse(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("mc_ms", C.c_double), ("normalise_ms", C.c_double), ("heat_ms", C.c_double),
                 ("arrhenius_ms", C.c_double), ("rebuild_ms", C.c_double),
-                ("heat_steps", C.c_double), ("heat_step_ms", C.c_double)]
+                ("heat_steps", C.c_double), ("heat_step_ms", C.c_double),
+                ("class_mode", C.c_double), ("tile", C.c_double), ("kz", C.c_double)]
 
 
 class TamError(RuntimeError):
@@ -117,6 +118,9 @@ def load():
     L.tam_energy.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.tam_get_timing.argtypes = [vp, C.POINTER(Timing)]
     L.tam_synchronize.argtypes = [vp]
+    L.tam_set_heat_launch.argtypes = [vp, C.c_int32, C.c_int32]
+    L.tam_tune_heat.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -225,3 +229,12 @@ class TissueGrid:
 
     def synchronize(self):
         self._chk(self.L.tam_synchronize(self.h))
+
+    def set_heat_launch(self, tile, kz):
+        self._chk(self.L.tam_set_heat_launch(self.h, int(tile), int(kz)))
+
+    def tune_heat(self, reps=5):
+        """Autotune the stencil launch shape; returns (tile, kz, ms per step)."""
+        t, k, ms = C.c_int32(), C.c_int32(), C.c_double()
+        self._chk(self.L.tam_tune_heat(self.h, int(reps), C.byref(t), C.byref(k), C.byref(ms)))
+        return t.value, k.value, ms.value
