@@ -154,6 +154,13 @@ int tam_set_thermal(tam_ctx* ctx, const double* temp, const double* kappa, const
 int tam_mc_run(tam_ctx* ctx, int64_t npackets, uint64_t seed, uint32_t stream, int32_t accumulate,
                tam_mc_stats* stats);
 
+/* --- transport launch shape (no reference counterpart: the reference runs
+ * one packet at a time per rank).  blocks_per_cu resident 256-lane blocks per
+ * CU set the packets in flight; tam_tune_mc times candidates on a sample of
+ * `sample` packets and keeps the fastest (the tally is cleared afterwards). */
+int tam_set_mc_launch(tam_ctx* ctx, int32_t blocks_per_cu);
+int tam_tune_mc(tam_ctx* ctx, int64_t sample, int32_t* blocks_per_cu, double* packets_per_s);
+
 /* --- normalisation: jmeanGLOBAL = mu_a * L * scale (mcpolar.f90:173-174);
  * scale = (getPwr()/81)/(nphotons*volumeVoxel) in the reference.  Also
  * forms the per-step heat source delt/(rho c) * jmean for the stencil. */

@@ -327,6 +327,7 @@ int tam_create(const tam_params* p, tam_ctx** out)
     if (e == hipSuccess) e = dalloc(&c->robin_lut, TAM_NCLASS);
     c->tile = 1;
     c->kz = 16;
+    c->mc_bpc = 8;
     // every stencil tile is at least 32 x 4 and kz >= 1
     c->neblk = ((nx + 31) / 32) * ((ny + 3) / 4) * nz;
     if (e == hipSuccess) e = dalloc(&c->eblk, c->neblk);
@@ -431,12 +432,21 @@ int tam_set_optics(tam_ctx* c, const uint8_t* mat, const tam_material* table, in
         tab[m].g = (float)t.g;
         tab[m].n = (float)t.n;
         tab[m].mua = (float)t.mua;
+        tab[m].omg2 = (float)(1.0 - t.g * t.g);
+        tab[m].opg2 = (float)(1.0 + t.g * t.g);
+        tab[m].inv2g = t.g != 0.0 ? (float)(1.0 / (2.0 * t.g)) : 0.f;
         frac[m] = mt > 0 ? t.mua / mt : 0.0;
         c->host_tab[m] = t;
     }
     c->air_id = nmat;                     // removed voxels: no absorption, no scattering, n = 1
     tab[nmat].albedo = 0.f; tab[nmat].g = 0.f; tab[nmat].n = 1.f; tab[nmat].mua = 0.f;
+    tab[nmat].omg2 = 1.f; tab[nmat].opg2 = 1.f; tab[nmat].inv2g = 0.f;
     frac[nmat] = 0.0;
+    for (int m = 0; m <= nmat; ++m) {     // normal-incidence specular reflectance at the top face
+        const float nt = (float)c->p.n_ext_zhi, ni = tab[m].n;
+        const float rr = (nt > 0.f && nt != ni) ? (ni - nt) / (ni + nt) : 0.f;
+        tab[m].rsp_top = rr * rr;
+    }
     c->host_tab[nmat] = tam_material{0, 0, 0, 1};
     c->nmat = nmat;
     std::vector<double> rk;
@@ -552,6 +562,7 @@ int tam_mc_run(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream, int
     a.tally = c->tally; a.counters = c->counters;
     a.p0 = 0; a.npk = (uint64_t)npackets;
     a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32); a.stream = stream;
+    a.blocks_per_cu = c->mc_bpc;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, launch_mc(a, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -801,6 +812,38 @@ int tam_get_timing(tam_ctx* c, tam_timing* out)
     out->class_mode = c->lut ? 1.0 : 0.0;
     out->tile = c->tile;
     out->kz = c->kz;
+    return TAM_OK;
+}
+
+int tam_set_mc_launch(tam_ctx* c, int32_t blocks_per_cu)
+{
+    if (!c) return TAM_ERR_ARG;
+    if (blocks_per_cu < 1 || blocks_per_cu > 64) return fail(c, TAM_ERR_ARG, "blocks_per_cu must be in [1, 64]");
+    c->mc_bpc = blocks_per_cu;
+    return TAM_OK;
+}
+
+int tam_tune_mc(tam_ctx* c, int64_t sample, int32_t* blocks_per_cu, double* packets_per_s)
+{
+    if (!c) return TAM_ERR_ARG;
+    if (sample < 1) return fail(c, TAM_ERR_ARG, "sample must be positive");
+    const int cands[] = {4, 8, 12, 16, 24, 32};
+    double best = 0.0;
+    int bb = c->mc_bpc;
+    for (int b : cands) {
+        c->mc_bpc = b;
+        tam_mc_stats st;
+        const int rc = tam_mc_run(c, sample, 0x5eed5eedULL, 0xffffffffu, 0, &st);
+        if (rc != TAM_OK) return rc;
+        const double rate = st.kernel_ms > 0 ? sample / (st.kernel_ms * 1e-3) : 0.0;
+        if (rate > best) { best = rate; bb = b; }
+    }
+    c->mc_bpc = bb;
+    Dev d(c->dev);
+    HIPCHK(c, launch_fill_f64(c->tally, c->nvox, 0.0, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (blocks_per_cu) *blocks_per_cu = bb;
+    if (packets_per_s) *packets_per_s = best;
     return TAM_OK;
 }
 

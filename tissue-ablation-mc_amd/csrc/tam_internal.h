@@ -23,6 +23,11 @@
 
 struct MatEntry {          // per material, in device memory
     float albedo, g, n, mua;
+    // Henyey-Greenstein constants and the specular reflectance of the top face
+    float omg2;            // 1 - g^2
+    float opg2;            // 1 + g^2
+    float inv2g;           // 1 / (2g), 0 when g == 0 (isotropic)
+    float rsp_top;         // ((n - n_top)/(n + n_top))^2, 0 when the top face is index matched
 };
 
 struct McArgs {
@@ -39,6 +44,7 @@ struct McArgs {
     unsigned long long* counters;   // TAM_CNT_*
     uint64_t p0, npk;
     uint32_t key0, key1, stream;
+    int blocks_per_cu;         // resident 256-lane blocks per CU (lanes = packets in flight)
 };
 
 enum { TAM_CNT_ABSORBED = 0, TAM_CNT_SPECULAR, TAM_CNT_ESC0, TAM_CNT_MISSED = TAM_CNT_ESC0 + 6,
@@ -160,6 +166,7 @@ struct tam_ctx {
     int tile, kz;               // stencil launch shape (autotuned)
     double* eblk;               // per-block energy slots
     long neblk;                 // slots: enough for the largest launch grid
+    int mc_bpc;                 // transport launch: blocks per CU (autotuned)
     float2* robin;
     double *kappa, *density, *heatcap, *coeff, *rhokap;
     float* mut;

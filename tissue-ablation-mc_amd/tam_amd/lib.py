@@ -26,7 +26,7 @@ HALO_FIELDS = {FIELD_TEMP, FIELD_RHOKAP, FIELD_KAPPA, FIELD_DENSITY, FIELD_HEATC
 EXPORTED = ("tam_abi_version", "tam_create", "tam_destroy", "tam_last_error", "tam_set_optics",
             "tam_set_thermal", "tam_mc_run", "tam_normalise", "tam_heat_run", "tam_arrhenius",
             "tam_rebuild", "tam_download", "tam_upload", "tam_energy", "tam_get_timing",
-            "tam_synchronize", "tam_set_heat_launch", "tam_tune_heat")
+            "tam_synchronize", "tam_set_heat_launch", "tam_tune_heat", "tam_set_mc_launch", "tam_tune_mc")
 
 
 class Params(C.Structure):
@@ -119,6 +119,8 @@ def load():
     L.tam_get_timing.argtypes = [vp, C.POINTER(Timing)]
     L.tam_synchronize.argtypes = [vp]
     L.tam_set_heat_launch.argtypes = [vp, C.c_int32, C.c_int32]
+    L.tam_set_mc_launch.argtypes = [vp, C.c_int32]
+    L.tam_tune_mc.argtypes = [vp, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
     L.tam_tune_heat.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                 C.POINTER(C.c_double)]
     _lib = L
@@ -232,6 +234,15 @@ class TissueGrid:
 
     def set_heat_launch(self, tile, kz):
         self._chk(self.L.tam_set_heat_launch(self.h, int(tile), int(kz)))
+
+    def set_mc_launch(self, blocks_per_cu):
+        self._chk(self.L.tam_set_mc_launch(self.h, int(blocks_per_cu)))
+
+    def tune_mc(self, sample=2_000_000):
+        """Autotune the transport launch; returns (blocks per CU, packets/s)."""
+        b, r = C.c_int32(), C.c_double()
+        self._chk(self.L.tam_tune_mc(self.h, int(sample), C.byref(b), C.byref(r)))
+        return b.value, r.value
 
     def tune_heat(self, reps=5):
         """Autotune the stencil launch shape; returns (tile, kz, ms per step)."""
