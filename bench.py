@@ -31,6 +31,17 @@ STENCIL_BYTES = 13.0                  # algorithmic bytes per voxel-update (T r/
 MC_OPS_CROSS, MC_OPS_INTER = 16.0, 64.0   # algorithmic FP32 lane-ops per crossing / interaction
 
 
+def measured_traffic(n):
+    """HBM bytes per stencil launch from the committed counter summary (rocprofv3
+    --pmc FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's reading
+    for the streaming copy of known size), or None where none was recorded."""
+    path = os.path.join(ROOT, "profiles", f"stencil_traffic_{n}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
 def cpu_baseline(prob, budget_s):
     """FP64 oracle on this host: MC packets/s and stencil voxel-updates/s on a
     bounded sample, combined into the same cycle's packets/s."""
@@ -108,6 +119,9 @@ def main():
     prob = configs.config1(n=(n, n, n), packets=int(args.packets), steps=args.heat_steps,
                            cycles=args.steps)
     run = CycleRunner(prob, seed=2024)
+    # measured streaming-copy bandwidth (256 MiB each way): the reference for
+    # the stencil's roofline and the calibration of the byte counters
+    copy_gbps = run.grid.copy_bandwidth(256 << 20, reps=20)
     if args.warmup > 0:
         run.run(args.warmup)
         run.reset()
@@ -123,7 +137,8 @@ def main():
     stencil = {"bound": "hbm", "achieved": STENCIL_BYTES * nvox / (step_ms * 1e-3) / 1e9,
                "peak": HBM_PEAK_GBS, "unit": "GB/s"}
     stencil["frac"] = stencil["achieved"] / stencil["peak"]
-    stencil["traffic"] = None
+    stencil["measured_copy_gbps"] = copy_gbps
+    stencil["traffic"] = measured_traffic(n)
     mc_ops = (MC_OPS_CROSS * T["crossings"] + MC_OPS_INTER * T["interactions"]) / mc_s
     mc_roof = {"bound": "valu", "achieved": mc_ops / 1e12, "peak": LANE_OPS_PEAK / 1e12,
                "unit": "Tlane-op/s", "frac": mc_ops / LANE_OPS_PEAK}
@@ -149,6 +164,9 @@ def main():
                       "interactions_per_packet": T["interactions"] / T["packets"]},
         "roofline": roofline,
         "roofline_mc": mc_roof,
+        "launch": {"stencil_tile": int(run.grid.timing().tile), "stencil_kz": int(run.grid.timing().kz),
+                   "class_mode": bool(run.grid.timing().class_mode),
+                   "mc_blocks_per_cu": run.tuned_mc[0] if run.tuned_mc else None},
         "cpu_baseline": None,
     }
     if mc_s > heat_s:

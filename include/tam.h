@@ -197,6 +197,10 @@ int tam_upload(tam_ctx* ctx, int32_t field, const double* in);
  * and the energy that left through the boundary faces since tam_set_thermal. */
 int tam_energy(tam_ctx* ctx, double* heat_content, double* boundary_out);
 int tam_get_timing(tam_ctx* ctx, tam_timing* out);
+/* Device-to-device streaming copy of `bytes` (rounded down to 16), `reps`
+ * times; *gbps = (read + written bytes) / mean kernel time.  A measured
+ * bandwidth reference for the stencil's roofline (no reference counterpart). */
+int tam_copy_bandwidth(tam_ctx* ctx, int64_t bytes, int32_t reps, double* gbps);
 int tam_synchronize(tam_ctx* ctx);
 
 #ifdef __cplusplus

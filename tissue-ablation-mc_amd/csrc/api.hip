@@ -815,6 +815,33 @@ int tam_get_timing(tam_ctx* c, tam_timing* out)
     return TAM_OK;
 }
 
+int tam_copy_bandwidth(tam_ctx* c, int64_t bytes, int32_t reps, double* gbps)
+{
+    if (!c || !gbps) return TAM_ERR_ARG;
+    if (bytes < 16 || reps < 1) return fail(c, TAM_ERR_ARG, "need bytes >= 16 and reps >= 1");
+    Dev d(c->dev);
+    bytes &= ~(int64_t)15;
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(c, hipMalloc(&a, (size_t)bytes));
+    hipError_t e = hipMalloc(&b, (size_t)bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(a);
+        return hip_fail(c, "hipMalloc", e);
+    }
+    e = hipMemsetAsync(a, 0, (size_t)bytes, c->stream);
+    if (e == hipSuccess) e = launch_copy(a, b, bytes, c->stream);          // warm-up
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy(a, b, bytes, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+    const double ms = elapsed(c) / reps;
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return hip_fail(c, "copy", e);
+    *gbps = 2.0 * (double)bytes / (ms * 1e-3) / 1e9;
+    return TAM_OK;
+}
+
 int tam_set_mc_launch(tam_ctx* c, int32_t blocks_per_cu)
 {
     if (!c) return TAM_ERR_ARG;

@@ -199,6 +199,13 @@ __global__ void fill_u64_kernel(unsigned long long* p, long n)
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
+// Streaming copy (16-byte loads and stores): the measured-bandwidth
+// reference beside the HBM spec, and the calibration for byte counters.
+__global__ __launch_bounds__(256) void copy_kernel(const float4* __restrict__ in, float4* __restrict__ out, long n4)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n4; i += (long)gridDim.x * 256) out[i] = in[i];
+}
+
 inline unsigned grid_for(long n) { return (unsigned)((n + 255) / 256); }
 inline unsigned grid_fill(long n)
 {
@@ -255,6 +262,16 @@ hipError_t launch_fill_f32(float* p, long n, float v, hipStream_t s)
     hipLaunchKernelGGL(fill_f32_kernel, dim3(grid_fill(n)), dim3(256), 0, s, p, n, v);
     return hipGetLastError();
 }
+hipError_t launch_copy(const void* in, void* out, long bytes, hipStream_t s)
+{
+    const long n4 = bytes / 16;
+    long b = (n4 + 255) / 256;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)b), dim3(256), 0, s, (const float4*)in, (float4*)out, n4);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_u64(unsigned long long* p, long n, hipStream_t s)
 {
     hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_fill(n)), dim3(256), 0, s, p, n);

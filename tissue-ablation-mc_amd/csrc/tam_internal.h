@@ -205,5 +205,6 @@ hipError_t launch_normalise(int nx, int ny, int nz, long px, long pxy, const dou
 hipError_t launch_fill_f64(double* p, long n, double v, hipStream_t s);
 hipError_t launch_fill_f32(float* p, long n, float v, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, long n, hipStream_t s);
+hipError_t launch_copy(const void* in, void* out, long bytes, hipStream_t s);
 hipError_t launch_energy(int nx, int ny, int nz, long px, long pxy, const float* dT, const double* T64,
                          double t_ref, const double* heatcap, double mass_voxel, double* out, hipStream_t s);

@@ -12,8 +12,9 @@ class CycleRunner:
         self.prob = prob
         self.seed = seed
         self.grid = prob.grid()
-        # stencil launch shape picked by timing the candidates on this grid
+        # launch shapes picked by timing the candidates on this grid
         self.tuned = self.grid.tune_heat(reps=5) if tune else None
+        self.tuned_mc = self.grid.tune_mc(min(int(prob.packets), 5_000_000)) if tune else None
         self.time = 0.0
         self.cycle = 0
         self.totals = dict(packets=0.0, crossings=0.0, interactions=0.0, mc_ms=0.0, heat_ms=0.0,

@@ -26,7 +26,8 @@ HALO_FIELDS = {FIELD_TEMP, FIELD_RHOKAP, FIELD_KAPPA, FIELD_DENSITY, FIELD_HEATC
 EXPORTED = ("tam_abi_version", "tam_create", "tam_destroy", "tam_last_error", "tam_set_optics",
             "tam_set_thermal", "tam_mc_run", "tam_normalise", "tam_heat_run", "tam_arrhenius",
             "tam_rebuild", "tam_download", "tam_upload", "tam_energy", "tam_get_timing",
-            "tam_synchronize", "tam_set_heat_launch", "tam_tune_heat", "tam_set_mc_launch", "tam_tune_mc")
+            "tam_synchronize", "tam_set_heat_launch", "tam_tune_heat", "tam_set_mc_launch", "tam_tune_mc",
+            "tam_copy_bandwidth")
 
 
 class Params(C.Structure):
@@ -120,6 +121,7 @@ def load():
     L.tam_synchronize.argtypes = [vp]
     L.tam_set_heat_launch.argtypes = [vp, C.c_int32, C.c_int32]
     L.tam_set_mc_launch.argtypes = [vp, C.c_int32]
+    L.tam_copy_bandwidth.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
     L.tam_tune_mc.argtypes = [vp, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
     L.tam_tune_heat.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                 C.POINTER(C.c_double)]
@@ -234,6 +236,12 @@ class TissueGrid:
 
     def set_heat_launch(self, tile, kz):
         self._chk(self.L.tam_set_heat_launch(self.h, int(tile), int(kz)))
+
+    def copy_bandwidth(self, nbytes=1 << 30, reps=10):
+        """Measured device copy bandwidth, GB/s (bytes read + written)."""
+        g = C.c_double()
+        self._chk(self.L.tam_copy_bandwidth(self.h, int(nbytes), int(reps), C.byref(g)))
+        return g.value
 
     def set_mc_launch(self, blocks_per_cu):
         self._chk(self.L.tam_set_mc_launch(self.h, int(blocks_per_cu)))
