@@ -133,6 +133,15 @@ int  tam_create(const tam_params* p, tam_ctx** out);
 void tam_destroy(tam_ctx* ctx);
 const char* tam_last_error(const tam_ctx* ctx);
 int  tam_abi_version(void);
+/* sizeof of the ABI structs, for FFI layout checks: 0 tam_params,
+ * 1 tam_material, 2 tam_mc_stats, 3 tam_pulse, 4 tam_timing; -1 otherwise. */
+int64_t tam_struct_size(int32_t which);
+
+/* One step of the laser pulse state machine (3dFD.f90:199-214): turn the
+ * laser off after realPulseLength, on again after repetitionRate_1 while
+ * pulses remain, then advance pulseCount, repetitionCount and time by delt.
+ * Host-only; tam_heat_run applies it after every step. */
+void tam_pulse_advance(tam_pulse* pulse, double delt);
 
 /* --- optics: init_opt1 + gridset (ch_opt.f90:7-25, gridset.f90:6-46) ----
  * mat: interior material ids (< nmat <= 255).  rhokap: optional halo array

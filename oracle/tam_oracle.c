@@ -376,17 +376,30 @@ static double rng_get(rng_t* r, int slot)
 /* HG deflection with the standard direction-cosine rotation (Wang, Jacques &
  * Zheng, MCML 1995, eqs. 3.22-3.27).  Same formula and 0.99999 threshold as
  * the GPU kernel. */
-static void rotate_hg(double* ux, double* uy, double* uz, double g, double xi_c, double xi_p)
+double or_hg_cos(double g, double xi)
 {
     double ct;
     if (g == 0.0) {
-        ct = 2.0 * xi_c - 1.0;
+        ct = 2.0 * xi - 1.0;
     } else {
-        double t = (1.0 - g * g) / (1.0 - g + 2.0 * g * xi_c);
+        double t = (1.0 - g * g) / (1.0 - g + 2.0 * g * xi);
         ct = (1.0 + g * g - t * t) / (2.0 * g);
     }
     if (ct > 1.0) ct = 1.0;
     if (ct < -1.0) ct = -1.0;
+    return ct;
+}
+
+void or_beam_sample(int beam, double radius, double u0, double u1, double xy[2])
+{
+    double r = beam == OR_BEAM_TOPHAT ? sqrt(u0 * (radius * radius)) : radius * sqrt(-log(u0) * 0.5);
+    xy[0] = r * cos(TWO_PI * u1);
+    xy[1] = r * sin(TWO_PI * u1);
+}
+
+static void rotate_hg(double* ux, double* uy, double* uz, double g, double xi_c, double xi_p)
+{
+    const double ct = or_hg_cos(g, xi_c);
     double st = sqrt(1.0 - ct * ct);
     double psi = TWO_PI * xi_p;
     double cp = cos(psi), sp = sin(psi);

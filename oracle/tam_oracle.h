@@ -125,6 +125,12 @@ int or_mc_run(const or_mc_t* m, uint64_t p0, uint64_t npk, double* tally_L,
 /* Fresnel reflectance, unpolarised (textbook), n_i -> n_t, cos of incidence. */
 double or_fresnel(double n_i, double n_t, double cos_i);
 
+/* The samplers or_mc_run uses, exposed for distribution tests:
+ * HG deflection cosine for uniform xi; beam launch offset (x, y) from the
+ * beam axis for uniforms (u0 radius, u1 azimuth). */
+double or_hg_cos(double g, double xi);
+void   or_beam_sample(int beam, double radius, double u0, double u1, double xy[2]);
+
 /* ------------------------------------------------------------ heat ------ */
 
 typedef struct {

@@ -275,6 +275,41 @@ extern "C" {
 
 int tam_abi_version(void) { return TAM_ABI_VERSION; }
 
+int64_t tam_struct_size(int32_t which)
+{
+    switch (which) {
+    case 0: return (int64_t)sizeof(tam_params);
+    case 1: return (int64_t)sizeof(tam_material);
+    case 2: return (int64_t)sizeof(tam_mc_stats);
+    case 3: return (int64_t)sizeof(tam_pulse);
+    case 4: return (int64_t)sizeof(tam_timing);
+    }
+    return -1;
+}
+
+void tam_pulse_advance(tam_pulse* p, double delt)
+{
+    if (!p) return;
+    if (p->use_pulse) {                                   // 3dFD.f90:199-210
+        if (p->pulseCount >= p->realPulseLength && p->laser_flag) {
+            p->laser_flag = 0;
+            p->laserOn = 0.;
+            p->pulseCount = 0.;
+            p->pulsesDone += 1;
+            p->repetitionCount = 0.;
+        } else if (p->repetitionCount >= p->repetitionRate_1 && !p->laser_flag &&
+                   p->pulsesDone < p->pulsesToDo) {
+            p->laser_flag = 1;
+            p->laserOn = 1.;
+            p->pulseCount = 0.;
+            p->repetitionCount = 0.;
+        }
+    }
+    p->pulseCount = p->pulseCount + delt;                 // 3dFD.f90:212-214
+    p->repetitionCount = p->repetitionCount + delt;
+    p->time = p->time + delt;
+}
+
 const char* tam_last_error(const tam_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
 
 int tam_create(const tam_params* p, tam_ctx** out)
@@ -621,28 +656,8 @@ int tam_heat_run(tam_ctx* c, int32_t loops, tam_pulse* pulse)
                                        c->tile, c->kz, c->stream));
             std::swap(c->cur, c->nxt);
         }
-        if (pulse) {
-            if (pulse->use_pulse) {                         // 3dFD.f90:199-210
-                if (pulse->pulseCount >= pulse->realPulseLength && pulse->laser_flag) {
-                    pulse->laser_flag = 0;
-                    pulse->laserOn = 0.;
-                    pulse->pulseCount = 0.;
-                    pulse->pulsesDone += 1;
-                    pulse->repetitionCount = 0.;
-                } else if (pulse->repetitionCount >= pulse->repetitionRate_1 && !pulse->laser_flag &&
-                           pulse->pulsesDone < pulse->pulsesToDo) {
-                    pulse->laser_flag = 1;
-                    pulse->laserOn = 1.;
-                    pulse->pulseCount = 0.;
-                    pulse->repetitionCount = 0.;
-                }
-            }
-            pulse->pulseCount = pulse->pulseCount + c->p.delt;     // :212-214
-            pulse->repetitionCount = pulse->repetitionCount + c->p.delt;
-            pulse->time = pulse->time + c->p.delt;
-        } else {
-            c->time += c->p.delt;
-        }
+        if (pulse) tam_pulse_advance(pulse, c->p.delt);
+        else c->time += c->p.delt;
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     int rc = check_flags(c);

@@ -27,7 +27,7 @@ EXPORTED = ("tam_abi_version", "tam_create", "tam_destroy", "tam_last_error", "t
             "tam_set_thermal", "tam_mc_run", "tam_normalise", "tam_heat_run", "tam_arrhenius",
             "tam_rebuild", "tam_download", "tam_upload", "tam_energy", "tam_get_timing",
             "tam_synchronize", "tam_set_heat_launch", "tam_tune_heat", "tam_set_mc_launch", "tam_tune_mc",
-            "tam_copy_bandwidth")
+            "tam_copy_bandwidth", "tam_struct_size", "tam_pulse_advance")
 
 
 class Params(C.Structure):
@@ -122,6 +122,10 @@ def load():
     L.tam_set_heat_launch.argtypes = [vp, C.c_int32, C.c_int32]
     L.tam_set_mc_launch.argtypes = [vp, C.c_int32]
     L.tam_copy_bandwidth.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
+    L.tam_struct_size.argtypes = [C.c_int32]
+    L.tam_struct_size.restype = C.c_int64
+    L.tam_pulse_advance.argtypes = [C.POINTER(Pulse), C.c_double]
+    L.tam_pulse_advance.restype = None
     L.tam_tune_mc.argtypes = [vp, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
     L.tam_tune_heat.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                 C.POINTER(C.c_double)]
