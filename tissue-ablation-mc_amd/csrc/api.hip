@@ -907,24 +907,50 @@ int tam_tune_heat(tam_ctx* c, int32_t reps, int32_t* tile, int32_t* kz, double* 
     double best = 1e300;
     int bt = c->tile, bk = c->kz;
     if (!c->fp64) {
-        const HeatArgs a = heat_args(c);
+        // Candidates run the real step (same flags as tam_heat_run) on the
+        // current field, cur -> nxt without swapping, so the state does not
+        // advance; the damage integral, threshold times and energy slots they
+        // update are scratch copies.
+        HeatArgs a = heat_args(c);
+        const long V = c->nvox;
+        double *om = nullptr, *th = nullptr, *eb = nullptr, *qs = nullptr;
+        HIPCHK(c, hipMalloc((void**)&om, sizeof(double) * (size_t)V));
+        hipError_t e = hipMalloc((void**)&th, sizeof(double) * 3 * (size_t)V);
+        if (e == hipSuccess) e = hipMalloc((void**)&eb, sizeof(double) * (size_t)c->neblk);
+        if (e == hipSuccess) e = hipMalloc((void**)&qs, sizeof(double) * (size_t)V);
+        if (e == hipSuccess) e = hipMemcpyAsync(om, c->omega, sizeof(double) * (size_t)V, hipMemcpyDeviceToDevice,
+                                                c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(qs, c->Q, sizeof(double) * (size_t)V, hipMemcpyDeviceToDevice,
+                                                c->stream);
+        if (e == hipSuccess) e = launch_fill_f64(th, 3 * V, 0.0, c->stream);
+        if (e == hipSuccess) e = launch_fill_f64(eb, c->neblk, 0.0, c->stream);
+        a.omega = om;
+        a.thres = th;
+        a.eblk = eb;
+        a.Q = qs;
+        const int boil = c->p.boil != 0, robin = c->p.robin_top != 0, dmg = c->p.damage_every_step != 0;
         const int kzs[] = {4, 8, 16, 32, 64};
-        for (int t = 0; t < heat_num_tiles(); ++t)
+        for (int t = 0; t < heat_num_tiles() && e == hipSuccess; ++t)
             for (int kk : kzs) {
                 if (kk > c->p.nz && kk != 4) continue;
-                // cur -> nxt without swapping: the state is not advanced; no
-                // damage, boiling or energy side effects
-                HIPCHK(c, launch_heat_step(a, c->cur, c->nxt, 1.f, 0.0, 0, c->p.robin_top, 0, 0, c->lut ? 1 : 0, t,
-                                           kk, c->stream));
-                HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-                for (int r = 0; r < reps; ++r)
-                    HIPCHK(c, launch_heat_step(a, c->cur, c->nxt, 1.f, 0.0, 0, c->p.robin_top, 0, 0,
-                                               c->lut ? 1 : 0, t, kk, c->stream));
-                HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-                HIPCHK(c, hipEventSynchronize(c->ev1));
+                e = launch_heat_step(a, c->cur, c->nxt, 1.f, 0.0, boil, robin, dmg, 1, c->lut ? 1 : 0, t, kk,
+                                     c->stream);
+                if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+                for (int r = 0; r < reps && e == hipSuccess; ++r)
+                    e = launch_heat_step(a, c->cur, c->nxt, 1.f, 0.0, boil, robin, dmg, 1, c->lut ? 1 : 0, t, kk,
+                                         c->stream);
+                if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+                if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+                if (e != hipSuccess) break;
                 const double ms = elapsed(c) / reps;
                 if (ms < best) { best = ms; bt = t; bk = kk; }
             }
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        (void)hipFree(om);
+        (void)hipFree(th);
+        (void)hipFree(eb);
+        (void)hipFree(qs);
+        if (e != hipSuccess) return hip_fail(c, "heat tuning", e);
         c->tile = bt;
         c->kz = bk;
     }

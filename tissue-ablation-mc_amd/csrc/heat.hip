@@ -35,6 +35,31 @@ __device__ __forceinline__ double wave_sum_d(double v)
 
 enum : unsigned { F_BOIL = 1u, F_ROBIN = 2u, F_DAMAGE = 4u, F_ENERGY = 8u };
 
+// Workgroups go to the 8 XCDs round-robin by linear id, and each XCD has its
+// own L2.  Tiles that are neighbours share halo cache lines, so the linear
+// block id is remapped to give every XCD one contiguous range of tiles
+// (x fastest, then y, then z): neighbours then meet in the same L2 instead of
+// each fetching the shared lines from HBM.  The map is a bijection on
+// [0, nblocks) for any block count.
+struct TileId {
+    int x, y, z;
+    long id;
+};
+__device__ __forceinline__ TileId xcd_tile()
+{
+    const long gx = gridDim.x, gy = gridDim.y;
+    const long nb = gx * gy * gridDim.z;
+    const long w = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const long q = nb / 8, r = nb % 8, x = w % 8, slot = w / 8;
+    const long t = x * q + (x < r ? x : r) + slot;
+    TileId tl;
+    tl.x = (int)(t % gx);
+    tl.y = (int)((t / gx) % gy);
+    tl.z = (int)(t / (gx * gy));
+    tl.id = t;
+    return tl;
+}
+
 template <int TX, int TY, bool LUT>
 __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const float* __restrict__ Tin,
                                                          float* __restrict__ Tout, float lon, double t_after,
@@ -56,17 +81,18 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
     const int nx = a.nx, ny = a.ny, nz = a.nz;
 
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int i = blockIdx.x * TX + tx + 1, j = blockIdx.y * TY + ty + 1;
+    const TileId tl = xcd_tile();
+    const int i = tl.x * TX + tx + 1, j = tl.y * TY + ty + 1;
     const bool in = i <= nx && j <= ny;
     const bool bnd_ij = i == 1 || i == nx || j == 1 || j == ny;     // lane on an x or y face
     // lanes past the grid edge load the halo cell (a valid address), never store
     const int ic = i <= nx + 1 ? i : nx + 1, jc = j <= ny + 1 ? j : ny + 1;
-    const int k0 = blockIdx.z * kz + 1;
+    const int k0 = tl.z * kz + 1;
     const int k1 = min(k0 + kz, nz + 1);
     const bool robin = flags & F_ROBIN, damage = flags & F_DAMAGE, boil = flags & F_BOIL;
     const bool energy = (flags & F_ENERGY) &&
-                        (blockIdx.x == 0 || blockIdx.x == gridDim.x - 1 || blockIdx.y == 0 ||
-                         blockIdx.y == gridDim.y - 1 || k0 == 1 || k1 == nz + 1);
+                        (tl.x == 0 || tl.x == (int)gridDim.x - 1 || tl.y == 0 ||
+                         tl.y == (int)gridDim.y - 1 || k0 == 1 || k1 == nz + 1);
     // halo duties of this lane
     const bool hxm = tx == 0, hxp = tx == TX - 1 && i <= nx, hym = ty == 0, hyp = ty == TY - 1 && j <= ny;
 
@@ -249,8 +275,7 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
         if (threadIdx.x == 0) {
             double t = 0.0;
             for (int q = 0; q < TX * TY / 64; ++q) t += sE[q];
-            const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
-            a.eblk[b] += t;
+            a.eblk[tl.id] += t;
         }
     }
 }
@@ -274,16 +299,17 @@ __global__ __launch_bounds__(TX* TY) void stencil_direct_kernel(const HeatArgs a
     const long px = a.px, pxy = a.pxy;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int i = blockIdx.x * TX + tx + 1, j = blockIdx.y * TY + ty + 1;
+    const TileId tl = xcd_tile();
+    const int i = tl.x * TX + tx + 1, j = tl.y * TY + ty + 1;
     const bool in = i <= nx && j <= ny;
     const bool bnd_ij = i == 1 || i == nx || j == 1 || j == ny;     // lane on an x or y face
     const int ic = i <= nx ? i : nx, jc = j <= ny ? j : ny;     // clamped: every neighbour is valid
-    const int k0 = blockIdx.z * kz + 1;
+    const int k0 = tl.z * kz + 1;
     const int k1 = min(k0 + kz, nz + 1);
     const bool robin = flags & F_ROBIN, damage = flags & F_DAMAGE, boil = flags & F_BOIL;
     const bool energy = (flags & F_ENERGY) &&
-                        (blockIdx.x == 0 || blockIdx.x == gridDim.x - 1 || blockIdx.y == 0 ||
-                         blockIdx.y == gridDim.y - 1 || k0 == 1 || k1 == nz + 1);
+                        (tl.x == 0 || tl.x == (int)gridDim.x - 1 || tl.y == 0 ||
+                         tl.y == (int)gridDim.y - 1 || k0 == 1 || k1 == nz + 1);
     if (LUT) {
         for (int t = threadIdx.x; t < 3 * 256 + 16; t += TX * TY) sLut[t] = a.glut[t];
         if (threadIdx.x < 16) sRob[threadIdx.x] = a.robin_lut[threadIdx.x];
@@ -390,8 +416,7 @@ __global__ __launch_bounds__(TX* TY) void stencil_direct_kernel(const HeatArgs a
         if (threadIdx.x == 0) {
             double t = 0.0;
             for (int q = 0; q < TX * TY / 64; ++q) t += sE[q];
-            const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
-            a.eblk[b] += t;
+            a.eblk[tl.id] += t;
         }
     }
 }

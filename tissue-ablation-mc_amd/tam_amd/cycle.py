@@ -37,6 +37,10 @@ class CycleRunner:
         p, g = self.prob, self.grid
         st = g.mc_run(p.packets, seed=self.seed, stream=self.cycle)
         g.normalise(p.scale)
+        if self.tuned is not None and p.steps >= 200:
+            # the damage window and the source change from cycle to cycle:
+            # re-time the stencil shapes on this cycle's field
+            self.tuned = g.tune_heat(reps=3)
         g.heat_run(p.steps)
         self.time += p.steps * p.params.delt
         if not p.params.damage_every_step:
