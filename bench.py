@@ -166,7 +166,8 @@ def main():
         "roofline_mc": mc_roof,
         "launch": {"stencil_tile": int(run.grid.timing().tile), "stencil_kz": int(run.grid.timing().kz),
                    "class_mode": bool(run.grid.timing().class_mode),
-                   "mc_blocks_per_cu": run.tuned_mc[0] if run.tuned_mc else None},
+                   "mc_blocks_per_cu": run.tuned_mc[0] if run.tuned_mc else None,
+                   "mc_rng_batch": run.tuned_mc[1] if run.tuned_mc else None},
         "cpu_baseline": None,
     }
     if mc_s > heat_s:

@@ -158,17 +158,23 @@ int tam_set_thermal(tam_ctx* ctx, const double* temp, const double* kappa, const
 
 /* --- photon loop: sourcephCO2 + tauint1 (+ stokes) for npackets packets,
  * replacing mcpolar.f90:151-170.  Tallies path length per voxel (FP64) into
- * the device tally; accumulate=0 clears it first.  Packet p draws from the
- * Philox4x32-10 stream key=seed, counter=(p, event, stream). */
+ * the device tally; accumulate=0 clears it first.  Voxels with mu_t = 0
+ * (ablated: mu_a = 0, so jmean is 0 there as in the reference) are crossed
+ * without a tally.  Packet p draws from the Philox4x32-10 stream key=seed,
+ * counter=(p, event, stream). */
 int tam_mc_run(tam_ctx* ctx, int64_t npackets, uint64_t seed, uint32_t stream, int32_t accumulate,
                tam_mc_stats* stats);
 
 /* --- transport launch shape (no reference counterpart: the reference runs
- * one packet at a time per rank).  blocks_per_cu resident 256-lane blocks per
- * CU set the packets in flight; tam_tune_mc times candidates on a sample of
- * `sample` packets and keeps the fastest (the tally is cleared afterwards). */
-int tam_set_mc_launch(tam_ctx* ctx, int32_t blocks_per_cu);
-int tam_tune_mc(tam_ctx* ctx, int64_t sample, int32_t* blocks_per_cu, double* packets_per_s);
+ * one packet at a time per rank).  blocks_per_cu 256-lane blocks per CU set
+ * the packets in flight; rng_batch (1..64) is how many lanes of a 64-lane
+ * wave must wait for random numbers before the wave draws them together.
+ * tam_tune_mc times candidate pairs on a sample of `sample` packets and keeps
+ * the fastest (the tally is cleared afterwards).  Neither changes results:
+ * packet p always draws from its own stream. */
+int tam_set_mc_launch(tam_ctx* ctx, int32_t blocks_per_cu, int32_t rng_batch);
+int tam_tune_mc(tam_ctx* ctx, int64_t sample, int32_t* blocks_per_cu, int32_t* rng_batch,
+                double* packets_per_s);
 
 /* --- normalisation: jmeanGLOBAL = mu_a * L * scale (mcpolar.f90:173-174);
  * scale = (getPwr()/81)/(nphotons*volumeVoxel) in the reference.  Also

@@ -485,7 +485,7 @@ static int mc_packet(const or_mc_t* m, const double* xf, const double* yf, const
             double taucell = dcell * rk;
             if (taurun + taucell < tau) {
                 taurun = taurun + taucell;
-                tally[v] += dcell;
+                if (rk > 0.) tally[v] += dcell;     /* mu_t = 0: nothing absorbed */
                 st->crossings += 1.0;
                 ref_update_pos(&w, g, xf, yf, zf, dcell, 1, dir, delta);
             } else {

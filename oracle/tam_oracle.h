@@ -116,7 +116,8 @@ typedef struct {
 } or_mc_stats_t;
 
 /* Runs packets [p0, p0+npk).  tally_L (nxg*nyg*nzg) accumulates path length
- * per voxel in cm (the absorbed fraction per voxel is mu_a * L).  With
+ * per voxel in cm (the absorbed fraction per voxel is mu_a * L); voxels with
+ * mu_t = 0 (ablated, mu_a = 0) are crossed without a tally.  With
  * OR_RNG_PHILOX each packet has its own counter stream, so the result does
  * not depend on the thread count; nthreads <= 0 uses OpenMP's default. */
 int or_mc_run(const or_mc_t* m, uint64_t p0, uint64_t npk, double* tally_L,

@@ -76,6 +76,60 @@ def test_mc_same_stream_follows_oracle():
     assert abs(st.interactions / ost.interactions - 1) < 1e-2
 
 
+def test_mc_same_stream_through_a_crater():
+    """A cylinder of mu_t = 0 voxels below the beam (an ablation crater): the
+    packets fly through it freely, neither side tallies there, and the rest
+    follows the oracle as in the intact case (same tolerances)."""
+    prob = _two_layer_problem(n=40, packets=200_000)
+    n = 40
+    mua = np.array([t[0] + t[1] for t in prob.table])
+    rk = np.zeros((n + 2,) * 3)
+    rk[1:-1, 1:-1, 1:-1] = mua[prob.mat]
+    zz, yy, xx = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    crater = ((xx - n / 2 + 0.5) ** 2 + (yy - n / 2 + 0.5) ** 2 <= 36) & (zz >= n - 12)
+    rk[1:-1, 1:-1, 1:-1][crater] = 0.0
+    prob.rhokap = rk
+    _print("case: two-layer 40^3 with a mu_t = 0 crater, 2e5 packets, seed 4321")
+    g = prob.grid()
+    st = g.mc_run(prob.packets, seed=4321)
+    t_gpu = g.download(lib.FIELD_TALLY)
+    g.close()
+    t_or, ost = _oracle_for(prob, 4321).run(prob.packets)
+    assert not t_gpu[crater].any() and not t_or[crater].any()
+    fg, fo = st.fractions(), _fractions_oracle(ost, prob.packets)
+    _print("gpu", fg, "\noracle", fo)
+    for k in fg:
+        assert abs(fg[k] - fo[k]) <= 1e-3, k
+    rel = np.abs(t_gpu - t_or).sum() / t_or.sum()
+    _print("tally L1 rel diff", rel)
+    assert rel < 1e-2
+    assert abs(st.crossings / ost.crossings - 1) < 1e-2
+
+
+def test_mc_launch_shape_does_not_change_results():
+    """Packets draw from their own streams whichever lane takes them: every
+    (blocks per CU, random-phase batch) gives the same integer fates and the
+    same tally up to FP64 summation order.  The tuner leaves the tally clear."""
+    prob = _two_layer_problem(n=32, packets=300_000)
+    g = prob.grid()
+    res = []
+    for bpc, batch in ((1, 1), (2, 64), (8, 32), (5, 13)):
+        _print(f"case: launch bpc={bpc} batch={batch}")
+        g.set_mc_launch(bpc, batch)
+        st = g.mc_run(prob.packets, seed=77, stream=3)
+        res.append((st, g.download(lib.FIELD_TALLY)))
+    b, t, rate = g.tune_mc(100_000)
+    assert 1 <= b <= 64 and 1 <= t <= 64 and rate > 0
+    assert not g.download(lib.FIELD_TALLY).any()
+    g.close()
+    st0, t0 = res[0]
+    for st, tl in res[1:]:
+        for k in ("absorbed", "specular", "missed", "crossings", "interactions", "killed"):
+            assert getattr(st, k) == getattr(st0, k), k
+        assert list(st.escaped) == list(st0.escaped)
+        np.testing.assert_allclose(tl, t0, rtol=1e-9, atol=1e-12)
+
+
 def _batches(g, nb, per, seed):
     out = []
     for b in range(nb):

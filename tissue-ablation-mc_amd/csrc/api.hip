@@ -318,6 +318,8 @@ int tam_create(const tam_params* p, tam_ctx** out)
     *out = nullptr;
     if (p->nx < 1 || p->ny < 1 || p->nz < 1 || p->nx > 4096 || p->ny > 4096 || p->nz > 4096)
         return fail(nullptr, TAM_ERR_ARG, "grid dimensions must be in [1, 4096]");
+    if ((double)p->nx * p->ny * p->nz >= 4294967296.0)
+        return fail(nullptr, TAM_ERR_ARG, "grid must hold fewer than 2^32 voxels (32-bit voxel index)");
     if (!(p->xmax > 0 && p->ymax > 0 && p->zmax > 0))
         return fail(nullptr, TAM_ERR_ARG, "grid extents must be positive");
     if (!(p->dx_m > 0 && p->dy_m > 0 && p->dz_m > 0 && p->delt > 0))
@@ -363,6 +365,7 @@ int tam_create(const tam_params* p, tam_ctx** out)
     c->tile = 1;
     c->kz = 16;
     c->mc_bpc = 8;
+    c->mc_batch = 32;
     // every stencil tile is at least 32 x 4 and kz >= 1
     c->neblk = ((nx + 31) / 32) * ((ny + 3) / 4) * nz;
     if (e == hipSuccess) e = dalloc(&c->eblk, c->neblk);
@@ -376,7 +379,7 @@ int tam_create(const tam_params* p, tam_ctx** out)
     if (e == hipSuccess) e = dalloc(&c->thres, 3 * V);
     if (e == hipSuccess) e = dalloc(&c->energy, 4);
     if (e == hipSuccess) e = dalloc(&c->flags, 4);
-    if (e == hipSuccess) e = dalloc(&c->counters, TAM_CNT_N);
+    if (e == hipSuccess) e = dalloc(&c->counters, TAM_CNT_SLOTS);
     if (c->fp64) {
         if (e == hipSuccess) e = dalloc(&c->T64a, H);
         if (e == hipSuccess) e = dalloc(&c->T64b, H);
@@ -581,7 +584,7 @@ int tam_mc_run(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream, int
     if (!c->optics_set) return fail(c, TAM_ERR_STATE, "tam_set_optics must come first");
     Dev d(c->dev);
     if (!accumulate) HIPCHK(c, launch_fill_f64(c->tally, c->nvox, 0.0, c->stream));
-    HIPCHK(c, launch_fill_u64(c->counters, TAM_CNT_N, c->stream));
+    HIPCHK(c, launch_fill_u64(c->counters, TAM_CNT_SLOTS, c->stream));
     McArgs a{};
     a.nx = c->p.nx; a.ny = c->p.ny; a.nz = c->p.nz;
     a.dx = (float)(2.0 * c->p.xmax / c->p.nx);
@@ -598,6 +601,7 @@ int tam_mc_run(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream, int
     a.p0 = 0; a.npk = (uint64_t)npackets;
     a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32); a.stream = stream;
     a.blocks_per_cu = c->mc_bpc;
+    a.rng_batch = c->mc_batch;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, launch_mc(a, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -857,34 +861,42 @@ int tam_copy_bandwidth(tam_ctx* c, int64_t bytes, int32_t reps, double* gbps)
     return TAM_OK;
 }
 
-int tam_set_mc_launch(tam_ctx* c, int32_t blocks_per_cu)
+int tam_set_mc_launch(tam_ctx* c, int32_t blocks_per_cu, int32_t rng_batch)
 {
     if (!c) return TAM_ERR_ARG;
     if (blocks_per_cu < 1 || blocks_per_cu > 64) return fail(c, TAM_ERR_ARG, "blocks_per_cu must be in [1, 64]");
+    if (rng_batch < 1 || rng_batch > 64) return fail(c, TAM_ERR_ARG, "rng_batch must be in [1, 64]");
     c->mc_bpc = blocks_per_cu;
+    c->mc_batch = rng_batch;
     return TAM_OK;
 }
 
-int tam_tune_mc(tam_ctx* c, int64_t sample, int32_t* blocks_per_cu, double* packets_per_s)
+int tam_tune_mc(tam_ctx* c, int64_t sample, int32_t* blocks_per_cu, int32_t* rng_batch, double* packets_per_s)
 {
     if (!c) return TAM_ERR_ARG;
     if (sample < 1) return fail(c, TAM_ERR_ARG, "sample must be positive");
-    const int cands[] = {4, 8, 12, 16, 24, 32};
+    const int bpcs[] = {2, 4, 6, 8};
+    const int batches[] = {8, 16, 24, 32, 40, 48};
     double best = 0.0;
-    int bb = c->mc_bpc;
-    for (int b : cands) {
-        c->mc_bpc = b;
-        tam_mc_stats st;
-        const int rc = tam_mc_run(c, sample, 0x5eed5eedULL, 0xffffffffu, 0, &st);
-        if (rc != TAM_OK) return rc;
-        const double rate = st.kernel_ms > 0 ? sample / (st.kernel_ms * 1e-3) : 0.0;
-        if (rate > best) { best = rate; bb = b; }
+    int bb = c->mc_bpc, bt = c->mc_batch;
+    for (int b : bpcs) {
+        for (int t : batches) {
+            c->mc_bpc = b;
+            c->mc_batch = t;
+            tam_mc_stats st;
+            const int rc = tam_mc_run(c, sample, 0x5eed5eedULL, 0xffffffffu, 0, &st);
+            if (rc != TAM_OK) return rc;
+            const double rate = st.kernel_ms > 0 ? sample / (st.kernel_ms * 1e-3) : 0.0;
+            if (rate > best) { best = rate; bb = b; bt = t; }
+        }
     }
     c->mc_bpc = bb;
+    c->mc_batch = bt;
     Dev d(c->dev);
     HIPCHK(c, launch_fill_f64(c->tally, c->nvox, 0.0, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (blocks_per_cu) *blocks_per_cu = bb;
+    if (rng_batch) *rng_batch = bt;
     if (packets_per_s) *packets_per_s = best;
     return TAM_OK;
 }

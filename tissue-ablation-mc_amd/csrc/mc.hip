@@ -5,26 +5,36 @@
 // inttau2.f90:7-239; HG scattering in place of stokes, stokes.f90:6-153).
 //
 // Design:
-//  * Each lane owns one packet at a time and takes packets p0+gid,
-//    p0+gid+stride, ... (static striding: no shared queue, so the packet ->
-//    random stream mapping does not depend on scheduling).  A lane whose
-//    packet ends launches the next one on its next iteration.
+//  * Each lane owns one packet at a time.  Packets are handed out from a
+//    queue head in device memory, a chunk per wave at a time, so waves that
+//    draw short histories take more packets and the grid drains together.
+//    The random stream of a packet depends only on its index, never on the
+//    lane or the order in which packets are taken.
 //  * Random numbers: Philox4x32-10, key = seed, counter = (packet id, event,
 //    stream).  One 4-word block per event: launch (radius, azimuth, specular
 //    test, first tau), interaction (albedo test, HG cos, HG azimuth, next
 //    tau), Fresnel test at a z face.  The FP64 oracle draws identically.
-//  * A wave executes the union of its lanes' paths, so every iteration has
-//    the same shape: geometry for live lanes, then ONE Philox block for every
-//    lane that needs random numbers (launching, interacting, or testing
-//    Fresnel reflection), then the event itself.
-//  * Voxel walk: integer cell + position local to the voxel (no nudge and
-//    no bisection as in update_pos/find): a crossing sets the crossed axis
-//    coordinate exactly to the entered face.  Voxels with mu_t = 0 (ablated)
-//    are crossed in free flight.  Distances use reciprocal direction
-//    components, refreshed when the direction changes.
+//  * Two phases.  A walk step moves every lane that needs no random numbers
+//    across one voxel face (or finds that its next interaction lies inside
+//    the current voxel, or that it reached a z face of the grid where a
+//    Fresnel test is due).  Such a lane then waits with its event pending.
+//    The random phase (one Philox block plus the launch, interaction or
+//    Fresnel event for every pending lane) runs once at least `rng_batch`
+//    lanes of the wave wait, or every lane with a packet does: the costly
+//    part then runs with most lanes active instead of for one or two.
+//  * Voxel walk (Amanatides-Woo DDA): per packet segment an origin o (cm,
+//    grid coordinates), the distance t travelled from it, and the distances
+//    Tx, Ty, Tz from o to the next face crossed on each axis.  A crossing
+//    takes the smallest T, adds one voxel's span on that axis to it, and
+//    steps the cell and the linear voxel index.  Scattering or a reflection
+//    starts a new segment.  Voxels with mu_t = 0 (ablated) are crossed in
+//    free flight.
 //  * Tally: path length is summed in a register while the packet stays in a
 //    voxel and written with one FP64 atomic add per voxel visit (at exit or
-//    absorption).  The absorbed fraction per voxel is mu_a * L.
+//    absorption).  The absorbed fraction per voxel is mu_a * L.  Voxels with
+//    mu_t = 0 absorb nothing and are crossed without a tally.  The global
+//    atomic rate (~2.4e10/s on MI355X whatever the address spread) is the
+//    bound of this kernel: without the tally it runs ~4.5x faster.
 //  * Transcendentals use the hardware approximations (v_log, v_sin/v_cos
 //    in revolutions, v_sqrt, v_rcp/v_rsq; ~1 ulp): they only shape sampled
 //    distances and directions, whose distributions the parity tests check.
@@ -80,211 +90,269 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
     return v;
 }
 
-constexpr float kInf = 3.0e38f;
-constexpr uint32_t kMaxEvents = 1u << 24;
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
-enum : int { EV_NONE = 0, EV_CROSS = 1, EV_INTERACT = 2 };
+// the wave's lowest active lane holds v; every lane gets it
+__device__ __forceinline__ uint64_t bcast_first(uint64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// distance from o (inside [lo, lo + d]) to the face crossed first along u
+__device__ __forceinline__ float face_dist(float o, float lo, float d, float u, float iu)
+{
+    return u > 0.f ? (lo + d - o) * iu : (u < 0.f ? (lo - o) * iu : 3.0e38f);
+}
+
+constexpr uint32_t kMaxEvents = 1u << 24;   // walk steps + random events of one packet
+constexpr uint32_t kChunk = 256;            // packets a wave takes from the queue at once
+
+enum : int { PEND_NONE = 0, PEND_LAUNCH = 1, PEND_INTERACT = 2, PEND_FRESNEL = 3 };
 
 __global__ __launch_bounds__(256) void mc_kernel(McArgs a)
 {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    uint64_t pid = a.p0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t pend = a.p0 + a.npk;
+    unsigned long long* head = a.counters + TAM_CNT_N;     // queue head, packets taken
+    const uint32_t nxy = (uint32_t)a.nx * (uint32_t)a.ny;
     const float idx_ = rcp(a.dx), idy_ = rcp(a.dy);
+    const float ztop = a.dz * (float)a.nz;
+    const uint32_t batch = (uint32_t)a.rng_batch;
 
     uint32_t c_abs = 0, c_spec = 0, c_miss = 0, c_cross = 0, c_inter = 0, c_kill = 0;
     uint32_t c_esc[6] = {0, 0, 0, 0, 0, 0};
 
-    bool alive = false;
+    // wave-uniform queue state: packets [pool, pool_end) are this wave's
+    uint64_t pool = 0, pool_end = 0;
+
+    bool have = false;                         // the lane holds a packet
+    int pend = PEND_NONE;
+    uint64_t pid = 0;
+    uint32_t ev = 0, nev = 0;                  // Philox event counter; steps of this packet
     int ci = 0, cj = 0, ck = 0;
-    float fx = 0.f, fy = 0.f, fz = 0.f;        // position inside the voxel, cm
-    float ux = 0.f, uy = 0.f, uz = -1.f;       // direction cosines
-    float iux = 0.f, iuy = 0.f, iuz = -1.f;    // their reciprocals (unused where u == 0)
+    uint32_t v = 0;                            // linear voxel index of (ci, cj, ck)
+    float mu = 0.f;                            // mu_t of voxel v
+    int mat = 0;                               // material of v, loaded when an event is pending
+    float ox = 0.f, oy = 0.f, oz = 0.f;        // segment origin, cm
+    float t = 0.f;                             // distance travelled from the origin
+    float Tx = 0.f, Ty = 0.f, Tz = 0.f;        // distance from the origin to the next face per axis
+    float ux = 0.f, uy = 0.f, uz = -1.f;
+    float iux = 0.f, iuy = 0.f, iuz = -1.f;    // reciprocals (unused where u == 0)
     float tau = 0.f, lacc = 0.f;               // optical depth left; path in this voxel
-    uint32_t ev = 0;                           // Philox event counter of this packet
-    uint32_t nev = 0;                          // events of this packet (exit guarantee)
+    float nin = 1.f, nout = 1.f;               // indices at a pending Fresnel test
 
     for (;;) {
-        if (!alive && pid >= pend) break;
-
-        // ---- geometry of live lanes: distance to the exit face vs. to the
-        //      next interaction (tauint1's `taurun + taucell < tau` test)
-        int evt = EV_NONE, axis = 0;
-        float s = 0.f, mu = 0.f;
-        long v = 0;
-        bool zexit = false, fres = false;
-        float nin = 1.f, nout = 0.f;
-        if (alive) {
-            v = ci + (long)a.nx * (cj + (long)a.ny * ck);
-            mu = a.mut[v];
-            const float sx = ux > 0.f ? (a.dx - fx) * iux : (ux < 0.f ? -fx * iux : kInf);
-            const float sy = uy > 0.f ? (a.dy - fy) * iuy : (uy < 0.f ? -fy * iuy : kInf);
-            const float sz = uz > 0.f ? (a.dz - fz) * iuz : (uz < 0.f ? -fz * iuz : kInf);
-            s = sx;
-            if (sy < s) { s = sy; axis = 1; }
-            if (sz < s) { s = sz; axis = 2; }
-            evt = (mu * s < tau) ? EV_CROSS : EV_INTERACT;
-            if (evt == EV_CROSS && axis == 2) {
-                const bool top = uz > 0.f;
-                zexit = top ? (ck == a.nz - 1) : (ck == 0);
-                if (zexit) {
-                    nout = top ? a.n_zhi : a.n_zlo;
-                    nin = a.mtab[a.mat[v]].n;
-                    fres = nout > 0.f && nout != nin;
-                }
-            }
-        }
-
-        // ---- one Philox block for every lane that needs random numbers
-        const bool need = !alive || evt == EV_INTERACT || fres;
-        uint4 r = make_uint4(0u, 0u, 0u, 0u);
-        if (need) {
-            r = philox4x32_10(make_uint4((uint32_t)pid, (uint32_t)(pid >> 32), ev, a.stream), a.key0, a.key1);
-            ++ev;
-        }
-
-        if (!alive) {
-            // ---- launch (sourcephCO2 with a beam profile), event 0
-            const float u0 = u01(r.x), u1 = u01(r.y), u2 = u01(r.z), u3 = u01(r.w);
-            const float rad = (a.beam == TAM_BEAM_TOPHAT) ? a.radius * fsqrt(u0)
-                                                           : a.radius * fsqrt(0.5f * neg_ln(u0));
-            float sn, cs;
-            sincos_rev(u1, &sn, &cs);
-            const float x = a.cx + rad * cs, y = a.cy + rad * sn;
-            ci = (int)floorf(x * idx_);
-            cj = (int)floorf(y * idy_);
-            ck = a.nz - 1;
-            const bool inside = x >= 0.f && y >= 0.f && ci >= 0 && ci < a.nx && cj >= 0 && cj < a.ny;
-            if (!inside) {
-                ++c_miss;
-                pid += stride;
-                ev = 0;
-                continue;
-            }
-            const long v0 = ci + (long)a.nx * (cj + (long)a.ny * ck);
-            if (u2 < a.mtab[a.mat[v0]].rsp_top) {
-                ++c_spec;
-                pid += stride;
-                ev = 0;
-                continue;
-            }
-            fx = fminf(fmaxf(x - ci * a.dx, 0.f), a.dx);
-            fy = fminf(fmaxf(y - cj * a.dy, 0.f), a.dy);
-            fz = a.dz;
-            ux = 0.f; uy = 0.f; uz = -1.f;
-            iux = 0.f; iuy = 0.f; iuz = -1.f;
-            tau = neg_ln(u3);
-            lacc = 0.f;
-            nev = 0;
-            alive = true;
-            continue;
-        }
-
-        if (++nev > kMaxEvents) {
-            // a packet that neither escapes nor is absorbed within kMaxEvents
-            // events (e.g. total internal reflection in a non-absorbing slab)
-            // is abandoned, so every lane's loop ends
-            ++c_kill;
-            alive = false;
-            pid += stride;
-            ev = 0;
-            continue;
-        }
-
-        if (evt == EV_CROSS) {
-            tau -= mu * s;
-            lacc += s;
-            ++c_cross;
-            unsafeAtomicAdd(&a.tally[v], (double)lacc);
-            lacc = 0.f;
-            bool out = false;
-            int face = 0;
-            if (axis == 0) {
-                fy = fminf(fmaxf(fy + s * uy, 0.f), a.dy);
-                fz = fminf(fmaxf(fz + s * uz, 0.f), a.dz);
-                if (ux > 0.f) { ++ci; fx = 0.f; } else { --ci; fx = a.dx; }
-                out = ci < 0 || ci >= a.nx;
-                face = ci < 0 ? TAM_FACE_XLO : TAM_FACE_XHI;
-            } else if (axis == 1) {
-                fx = fminf(fmaxf(fx + s * ux, 0.f), a.dx);
-                fz = fminf(fmaxf(fz + s * uz, 0.f), a.dz);
-                if (uy > 0.f) { ++cj; fy = 0.f; } else { --cj; fy = a.dy; }
-                out = cj < 0 || cj >= a.ny;
-                face = cj < 0 ? TAM_FACE_YLO : TAM_FACE_YHI;
+        // ---- lanes without a packet take the next ones from the queue
+        const uint64_t wm = ballot(!have);
+        if (wm) {
+            const uint32_t cnt = (uint32_t)__popcll(wm);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(wm >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)wm, 0u));
+            const uint64_t avail = pool_end - pool;
+            uint64_t mine = pool + rank;
+            if (avail < cnt) {
+                uint64_t nb = 0;
+                if (rank == 0 && !have) nb = atomicAdd(head, (unsigned long long)kChunk);
+                // the lowest lane wanting a packet is not always the lowest
+                // active lane: pass the chunk base through the wave
+                nb = __shfl(nb, (int)(__ffsll((long long)wm) - 1), 64);
+                nb = bcast_first(nb);
+                if (rank >= avail) mine = nb + (rank - avail);
+                pool = nb + (cnt - avail);
+                pool_end = nb + kChunk;
+                if (pool_end > a.npk) pool_end = a.npk > nb ? a.npk : nb;
+                if (pool > pool_end) pool = pool_end;
             } else {
-                fx = fminf(fmaxf(fx + s * ux, 0.f), a.dx);
-                fy = fminf(fmaxf(fy + s * uy, 0.f), a.dy);
-                const bool top = uz > 0.f;
-                if (zexit) {
-                    // z face of the grid: Fresnel test, else escape
-                    if (fres && u01(r.x) < fresnel(nin, nout, fabsf(uz))) {
-                        fz = top ? a.dz : 0.f;
-                        uz = -uz;
-                        iuz = -iuz;
+                pool += cnt;
+            }
+            if (!have) {
+                if (mine >= a.npk) break;          // queue drained: this lane is done
+                have = true;
+                pend = PEND_LAUNCH;
+                pid = a.p0 + mine;
+                ev = 0;
+                nev = 0;
+            }
+        }
+
+        const uint64_t am = ballot(true);
+        const uint64_t pm = ballot(pend != PEND_NONE);
+        if ((uint32_t)__popcll(pm) >= batch || pm == am) {
+            // ================= random phase: lanes with a pending event ====
+            if (pend != PEND_NONE) {
+                const uint4 r = philox4x32_10(make_uint4((uint32_t)pid, (uint32_t)(pid >> 32), ev, a.stream),
+                                              a.key0, a.key1);
+                ++ev;
+                const int kind = pend;
+                pend = PEND_NONE;
+                if (kind == PEND_LAUNCH) {
+                    // sourcephCO2 with a beam profile (sourceph.f90:7-49)
+                    const float u0 = u01(r.x), u1 = u01(r.y), u2 = u01(r.z), u3 = u01(r.w);
+                    const float rad = (a.beam == TAM_BEAM_TOPHAT) ? a.radius * fsqrt(u0)
+                                                                   : a.radius * fsqrt(0.5f * neg_ln(u0));
+                    float sn, cs;
+                    sincos_rev(u1, &sn, &cs);
+                    const float x = a.cx + rad * cs, y = a.cy + rad * sn;
+                    ci = (int)floorf(x * idx_);
+                    cj = (int)floorf(y * idy_);
+                    ck = a.nz - 1;
+                    const bool inside = x >= 0.f && y >= 0.f && ci >= 0 && ci < a.nx && cj >= 0 && cj < a.ny;
+                    if (!inside) {
+                        ++c_miss;
+                        have = false;
                     } else {
-                        out = true;
-                        face = top ? TAM_FACE_ZHI : TAM_FACE_ZLO;
+                        v = (uint32_t)ci + (uint32_t)a.nx * (uint32_t)cj + nxy * (uint32_t)ck;
+                        if (u2 < a.mtab[a.mat[v]].rsp_top) {
+                            ++c_spec;
+                            have = false;
+                        } else {
+                            const float xl = (float)ci * a.dx, yl = (float)cj * a.dy;
+                            ox = fminf(fmaxf(x, xl), xl + a.dx);
+                            oy = fminf(fmaxf(y, yl), yl + a.dy);
+                            oz = ztop;
+                            ux = 0.f; uy = 0.f; uz = -1.f;
+                            iux = 0.f; iuy = 0.f; iuz = -1.f;
+                            t = 0.f;
+                            Tx = 3.0e38f; Ty = 3.0e38f; Tz = a.dz;
+                            mu = a.mut[v];
+                            tau = neg_ln(u3);
+                            lacc = 0.f;
+                        }
+                    }
+                } else if (kind == PEND_INTERACT) {
+                    // interaction inside voxel v, tau / mu beyond the current point
+                    const float s = tau * rcp(mu);
+                    lacc += s;
+                    const float tt = t + s;
+                    const float xl = (float)ci * a.dx, yl = (float)cj * a.dy, zl = (float)ck * a.dz;
+                    ox = fminf(fmaxf(ox + tt * ux, xl), xl + a.dx);
+                    oy = fminf(fmaxf(oy + tt * uy, yl), yl + a.dy);
+                    oz = fminf(fmaxf(oz + tt * uz, zl), zl + a.dz);
+                    ++c_inter;
+                    const MatEntry me = a.mtab[mat];
+                    if (!(u01(r.x) < me.albedo)) {
+                        unsafeAtomicAdd(&a.tally[v], (double)lacc);
+                        lacc = 0.f;
+                        ++c_abs;
+                        have = false;
+                    } else {
+                        // Henyey-Greenstein deflection, MCML direction-cosine update
+                        float ct;
+                        if (me.inv2g == 0.f) {
+                            ct = 2.f * u01(r.y) - 1.f;
+                        } else {
+                            const float q = me.omg2 * rcp(1.f - me.g + 2.f * me.g * u01(r.y));
+                            ct = (me.opg2 - q * q) * me.inv2g;
+                        }
+                        ct = fminf(fmaxf(ct, -1.f), 1.f);
+                        const float st = fsqrt(1.f - ct * ct);
+                        float sp, cp;
+                        sincos_rev(u01(r.z), &sp, &cp);
+                        float nx_, ny_, nz_;
+                        if (fabsf(uz) > 0.99999f) {
+                            nx_ = st * cp;
+                            ny_ = st * sp;
+                            nz_ = uz >= 0.f ? ct : -ct;
+                        } else {
+                            const float w2 = 1.f - uz * uz;
+                            const float rt = __builtin_amdgcn_rsqf(w2);    // 1/sqrt(1-uz^2)
+                            nx_ = st * (ux * uz * cp - uy * sp) * rt + ux * ct;
+                            ny_ = st * (uy * uz * cp + ux * sp) * rt + uy * ct;
+                            nz_ = -st * cp * (w2 * rt) + uz * ct;
+                        }
+                        const float inv = __builtin_amdgcn_rsqf(nx_ * nx_ + ny_ * ny_ + nz_ * nz_);
+                        ux = nx_ * inv; uy = ny_ * inv; uz = nz_ * inv;
+                        iux = rcp(ux); iuy = rcp(uy); iuz = rcp(uz);
+                        t = 0.f;
+                        Tx = face_dist(ox, xl, a.dx, ux, iux);
+                        Ty = face_dist(oy, yl, a.dy, uy, iuy);
+                        Tz = face_dist(oz, zl, a.dz, uz, iuz);
+                        tau = neg_ln(u01(r.w));
                     }
                 } else {
-                    if (top) { ++ck; fz = 0.f; } else { --ck; fz = a.dz; }
+                    // Fresnel test at a z face of the grid (the crossing is tallied)
+                    const bool top = uz > 0.f;
+                    if (u01(r.x) < fresnel(nin, nout, fabsf(uz))) {
+                        // reflect: new segment from the point on the face
+                        const float xl = (float)ci * a.dx, yl = (float)cj * a.dy, zl = (float)ck * a.dz;
+                        ox = fminf(fmaxf(ox + t * ux, xl), xl + a.dx);
+                        oy = fminf(fmaxf(oy + t * uy, yl), yl + a.dy);
+                        oz = top ? zl + a.dz : zl;
+                        Tx -= t;
+                        Ty -= t;
+                        t = 0.f;
+                        uz = -uz;
+                        iuz = -iuz;
+                        Tz = a.dz * fabsf(iuz);
+                    } else {
+                        ++c_esc[top ? TAM_FACE_ZHI : TAM_FACE_ZLO];
+                        have = false;
+                    }
                 }
             }
-            if (out) {
-                ++c_esc[face];
-                alive = false;
-                pid += stride;
-                ev = 0;
+        } else if (pend == PEND_NONE) {
+            // ================= walk phase: one face crossing per lane ======
+            if (++nev > kMaxEvents) {
+                // a packet that neither escapes nor is absorbed within
+                // kMaxEvents steps (e.g. total internal reflection in a
+                // non-absorbing slab) is abandoned, so every lane's loop ends
+                ++c_kill;
+                have = false;
+                continue;
             }
-        } else {
-            // interaction inside the voxel
-            const float t = tau * rcp(mu);
-            lacc += t;
-            fx = fminf(fmaxf(fx + t * ux, 0.f), a.dx);
-            fy = fminf(fmaxf(fy + t * uy, 0.f), a.dy);
-            fz = fminf(fmaxf(fz + t * uz, 0.f), a.dz);
-            ++c_inter;
-            const MatEntry me = a.mtab[a.mat[v]];
-            if (!(u01(r.x) < me.albedo)) {
-                unsafeAtomicAdd(&a.tally[v], (double)lacc);
+            int axis = 0;
+            float tn = Tx;
+            if (Ty < tn) { tn = Ty; axis = 1; }
+            if (Tz < tn) { tn = Tz; axis = 2; }
+            const float s = fmaxf(tn - t, 0.f);
+            if (mu * s < tau) {
+                tau -= mu * s;
+                lacc += s;
+                ++c_cross;
+                // mu_t = 0 (ablated): nothing is absorbed, and the atomic
+                // unit's throughput is what bounds this kernel
+                if (mu > 0.f) unsafeAtomicAdd(&a.tally[v], (double)lacc);
                 lacc = 0.f;
-                ++c_abs;
-                alive = false;
-                pid += stride;
-                ev = 0;
+                t = tn;
+                bool out = false;
+                int face = 0;
+                if (axis == 0) {
+                    Tx += a.dx * fabsf(iux);
+                    if (ux > 0.f) { ++ci; ++v; out = ci >= a.nx; face = TAM_FACE_XHI; }
+                    else          { --ci; --v; out = ci < 0;     face = TAM_FACE_XLO; }
+                } else if (axis == 1) {
+                    Ty += a.dy * fabsf(iuy);
+                    if (uy > 0.f) { ++cj; v += a.nx; out = cj >= a.ny; face = TAM_FACE_YHI; }
+                    else          { --cj; v -= a.nx; out = cj < 0;     face = TAM_FACE_YLO; }
+                } else {
+                    const bool top = uz > 0.f;
+                    if (top ? (ck == a.nz - 1) : (ck == 0)) {
+                        // z face of the grid: Fresnel test where the index changes
+                        nout = top ? a.n_zhi : a.n_zlo;
+                        nin = a.mtab[a.mat[v]].n;
+                        if (nout > 0.f && nout != nin) pend = PEND_FRESNEL;
+                        else { out = true; face = top ? TAM_FACE_ZHI : TAM_FACE_ZLO; }
+                    } else {
+                        Tz += a.dz * fabsf(iuz);
+                        if (top) { ++ck; v += nxy; } else { --ck; v -= nxy; }
+                    }
+                }
+                if (out) {
+                    ++c_esc[face];
+                    have = false;
+                } else if (pend == PEND_NONE) {
+                    mu = a.mut[v];
+                }
             } else {
-                // Henyey-Greenstein deflection, MCML direction-cosine update
-                float ct;
-                if (me.inv2g == 0.f) {
-                    ct = 2.f * u01(r.y) - 1.f;
-                } else {
-                    const float q = me.omg2 * rcp(1.f - me.g + 2.f * me.g * u01(r.y));
-                    ct = (me.opg2 - q * q) * me.inv2g;
-                }
-                ct = fminf(fmaxf(ct, -1.f), 1.f);
-                const float st = fsqrt(1.f - ct * ct);
-                float sp, cp;
-                sincos_rev(u01(r.z), &sp, &cp);
-                float nx_, ny_, nz_;
-                if (fabsf(uz) > 0.99999f) {
-                    nx_ = st * cp;
-                    ny_ = st * sp;
-                    nz_ = uz >= 0.f ? ct : -ct;
-                } else {
-                    const float w2 = 1.f - uz * uz;
-                    const float rt = __builtin_amdgcn_rsqf(w2);    // 1/sqrt(1-uz^2)
-                    nx_ = st * (ux * uz * cp - uy * sp) * rt + ux * ct;
-                    ny_ = st * (uy * uz * cp + ux * sp) * rt + uy * ct;
-                    nz_ = -st * cp * (w2 * rt) + uz * ct;
-                }
-                const float inv = __builtin_amdgcn_rsqf(nx_ * nx_ + ny_ * ny_ + nz_ * nz_);
-                ux = nx_ * inv; uy = ny_ * inv; uz = nz_ * inv;
-                iux = rcp(ux); iuy = rcp(uy); iuz = rcp(uz);
-                tau = neg_ln(u01(r.w));
+                pend = PEND_INTERACT;
+                mat = a.mat[v];
             }
         }
     }
 
     // ---- per-wave reduction of the fate and work counters ----------------
+    // (lanes leave the loop one by one as the queue drains; the reduction
+    // runs once every lane of the wave has left)
     const uint32_t vals[TAM_CNT_N] = {c_abs, c_spec, c_esc[0], c_esc[1], c_esc[2], c_esc[3],
                                       c_esc[4], c_esc[5], c_miss, c_cross, c_inter, c_kill};
     const bool leader = (threadIdx.x & 63) == 0;
@@ -304,8 +372,9 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess || ncu <= 0) ncu = 256;
-    // blocks_per_cu resident 256-lane blocks per CU (autotuned); never more
-    // lanes than packets
+    // blocks_per_cu 256-lane blocks per CU (autotuned); every lane takes
+    // packets from the queue until it is empty, so more blocks than packets
+    // only idle
     long blocks = (long)ncu * (a.blocks_per_cu > 0 ? a.blocks_per_cu : 8);
     const long need = (long)((a.npk + 255) / 256);
     if (need < blocks) blocks = need;

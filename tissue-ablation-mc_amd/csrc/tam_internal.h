@@ -44,11 +44,13 @@ struct McArgs {
     unsigned long long* counters;   // TAM_CNT_*
     uint64_t p0, npk;
     uint32_t key0, key1, stream;
-    int blocks_per_cu;         // resident 256-lane blocks per CU (lanes = packets in flight)
+    int blocks_per_cu;         // 256-lane blocks per CU (lanes = packets in flight)
+    int rng_batch;             // pending lanes of a wave that start its random phase
 };
 
+// counters[TAM_CNT_N] is the transport's packet queue head
 enum { TAM_CNT_ABSORBED = 0, TAM_CNT_SPECULAR, TAM_CNT_ESC0, TAM_CNT_MISSED = TAM_CNT_ESC0 + 6,
-       TAM_CNT_CROSS, TAM_CNT_INTER, TAM_CNT_KILLED, TAM_CNT_N };
+       TAM_CNT_CROSS, TAM_CNT_INTER, TAM_CNT_KILLED, TAM_CNT_N, TAM_CNT_SLOTS };
 
 struct HeatArgs {
     int nx, ny, nz;
@@ -167,6 +169,7 @@ struct tam_ctx {
     double* eblk;               // per-block energy slots
     long neblk;                 // slots: enough for the largest launch grid
     int mc_bpc;                 // transport launch: blocks per CU (autotuned)
+    int mc_batch;               // transport launch: random-phase batch (autotuned)
     float2* robin;
     double *kappa, *density, *heatcap, *coeff, *rhokap;
     float* mut;

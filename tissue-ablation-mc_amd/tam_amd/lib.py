@@ -120,13 +120,13 @@ def load():
     L.tam_get_timing.argtypes = [vp, C.POINTER(Timing)]
     L.tam_synchronize.argtypes = [vp]
     L.tam_set_heat_launch.argtypes = [vp, C.c_int32, C.c_int32]
-    L.tam_set_mc_launch.argtypes = [vp, C.c_int32]
+    L.tam_set_mc_launch.argtypes = [vp, C.c_int32, C.c_int32]
     L.tam_copy_bandwidth.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
     L.tam_struct_size.argtypes = [C.c_int32]
     L.tam_struct_size.restype = C.c_int64
     L.tam_pulse_advance.argtypes = [C.POINTER(Pulse), C.c_double]
     L.tam_pulse_advance.restype = None
-    L.tam_tune_mc.argtypes = [vp, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+    L.tam_tune_mc.argtypes = [vp, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_double)]
     L.tam_tune_heat.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                 C.POINTER(C.c_double)]
     _lib = L
@@ -247,14 +247,15 @@ class TissueGrid:
         self._chk(self.L.tam_copy_bandwidth(self.h, int(nbytes), int(reps), C.byref(g)))
         return g.value
 
-    def set_mc_launch(self, blocks_per_cu):
-        self._chk(self.L.tam_set_mc_launch(self.h, int(blocks_per_cu)))
+    def set_mc_launch(self, blocks_per_cu, rng_batch=32):
+        self._chk(self.L.tam_set_mc_launch(self.h, int(blocks_per_cu), int(rng_batch)))
 
     def tune_mc(self, sample=2_000_000):
-        """Autotune the transport launch; returns (blocks per CU, packets/s)."""
-        b, r = C.c_int32(), C.c_double()
-        self._chk(self.L.tam_tune_mc(self.h, int(sample), C.byref(b), C.byref(r)))
-        return b.value, r.value
+        """Autotune the transport launch; returns (blocks per CU, random-phase
+        batch, packets/s)."""
+        b, t, r = C.c_int32(), C.c_int32(), C.c_double()
+        self._chk(self.L.tam_tune_mc(self.h, int(sample), C.byref(b), C.byref(t), C.byref(r)))
+        return b.value, t.value, r.value
 
     def tune_heat(self, reps=5):
         """Autotune the stencil launch shape; returns (tile, kz, ms per step)."""
