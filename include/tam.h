@@ -110,6 +110,8 @@ typedef struct {
     double crossings, interactions;                  /* work counters */
     double kernel_ms;                                /* device time of the transport kernel */
     double killed;                                   /* abandoned after 2^24 events (exit guarantee) */
+    double tally_adds;                               /* FP64 atomic adds into the tally (one per
+                                                        voxel visit with mu_t > 0) */
 } tam_mc_stats;
 
 /* Pulse state (3dFD.f90:7-12, :199-214); advanced by tam_heat_run. */
@@ -216,6 +218,11 @@ int tam_get_timing(tam_ctx* ctx, tam_timing* out);
  * times; *gbps = (read + written bytes) / mean kernel time.  A measured
  * bandwidth reference for the stencil's roofline (no reference counterpart). */
 int tam_copy_bandwidth(tam_ctx* ctx, int64_t bytes, int32_t reps, double* gbps);
+/* FP64 atomic adds without return, `count` of them (rounded to whole waves)
+ * to random addresses over a scratch array of `slots` doubles (at least
+ * 64); *per_s = adds / kernel time.  The measured ceiling of the transport's
+ * tally (no reference counterpart). */
+int tam_atomic_rate(tam_ctx* ctx, int64_t count, int64_t slots, double* per_s);
 int tam_synchronize(tam_ctx* ctx);
 
 #ifdef __cplusplus

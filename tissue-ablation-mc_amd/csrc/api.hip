@@ -620,6 +620,7 @@ int tam_mc_run(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream, int
         stats->crossings = (double)cnt[TAM_CNT_CROSS];
         stats->interactions = (double)cnt[TAM_CNT_INTER];
         stats->killed = (double)cnt[TAM_CNT_KILLED];
+        stats->tally_adds = (double)cnt[TAM_CNT_TALLY];
         stats->kernel_ms = c->timing.mc_ms;
     }
     return TAM_OK;
@@ -858,6 +859,29 @@ int tam_copy_bandwidth(tam_ctx* c, int64_t bytes, int32_t reps, double* gbps)
     (void)hipFree(b);
     if (e != hipSuccess) return hip_fail(c, "copy", e);
     *gbps = 2.0 * (double)bytes / (ms * 1e-3) / 1e9;
+    return TAM_OK;
+}
+
+int tam_atomic_rate(tam_ctx* c, int64_t count, int64_t slots, double* per_s)
+{
+    if (!c || !per_s) return TAM_ERR_ARG;
+    if (count < 1 || slots < 64 || slots > 0xffffffffLL)
+        return fail(c, TAM_ERR_ARG, "need count >= 1 and 64 <= slots < 2^32");
+    Dev d(c->dev);
+    double* t = nullptr;
+    HIPCHK(c, hipMalloc(&t, (size_t)slots * sizeof(double)));
+    const long lanes = 2048L * 256L;
+    const long per = count / lanes < 1 ? 1 : count / lanes;
+    hipError_t e = launch_fill_f64(t, slots, 0.0, c->stream);
+    if (e == hipSuccess) e = launch_atomic_rate(t, slots, lanes, c->stream);        // warm-up
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    if (e == hipSuccess) e = launch_atomic_rate(t, slots, per * lanes, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+    const double ms = elapsed(c);
+    (void)hipFree(t);
+    if (e != hipSuccess) return hip_fail(c, "atomic rate", e);
+    *per_s = (double)(per * lanes) / (ms * 1e-3);
     return TAM_OK;
 }
 

@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void mc_kernel(McArgs a)
     const float ztop = a.dz * (float)a.nz;
     const uint32_t batch = (uint32_t)a.rng_batch;
 
-    uint32_t c_abs = 0, c_spec = 0, c_miss = 0, c_cross = 0, c_inter = 0, c_kill = 0;
+    uint32_t c_abs = 0, c_spec = 0, c_miss = 0, c_cross = 0, c_inter = 0, c_kill = 0, c_tally = 0;
     uint32_t c_esc[6] = {0, 0, 0, 0, 0, 0};
 
     // wave-uniform queue state: packets [pool, pool_end) are this wave's
@@ -232,6 +232,7 @@ __global__ __launch_bounds__(256) void mc_kernel(McArgs a)
                     const MatEntry me = a.mtab[mat];
                     if (!(u01(r.x) < me.albedo)) {
                         unsafeAtomicAdd(&a.tally[v], (double)lacc);
+                        ++c_tally;
                         lacc = 0.f;
                         ++c_abs;
                         have = false;
@@ -311,7 +312,10 @@ __global__ __launch_bounds__(256) void mc_kernel(McArgs a)
                 ++c_cross;
                 // mu_t = 0 (ablated): nothing is absorbed, and the atomic
                 // unit's throughput is what bounds this kernel
-                if (mu > 0.f) unsafeAtomicAdd(&a.tally[v], (double)lacc);
+                if (mu > 0.f) {
+                    unsafeAtomicAdd(&a.tally[v], (double)lacc);
+                    ++c_tally;
+                }
                 lacc = 0.f;
                 t = tn;
                 bool out = false;
@@ -354,7 +358,8 @@ __global__ __launch_bounds__(256) void mc_kernel(McArgs a)
     // (lanes leave the loop one by one as the queue drains; the reduction
     // runs once every lane of the wave has left)
     const uint32_t vals[TAM_CNT_N] = {c_abs, c_spec, c_esc[0], c_esc[1], c_esc[2], c_esc[3],
-                                      c_esc[4], c_esc[5], c_miss, c_cross, c_inter, c_kill};
+                                      c_esc[4], c_esc[5], c_miss, c_cross, c_inter, c_kill,
+                                      c_tally};
     const bool leader = (threadIdx.x & 63) == 0;
 #pragma unroll
     for (int i = 0; i < TAM_CNT_N; ++i) {
@@ -363,7 +368,30 @@ __global__ __launch_bounds__(256) void mc_kernel(McArgs a)
     }
 }
 
+// Atomic-rate probe: each lane adds 1.0 to `per_lane` xorshift-chosen slots.
+__global__ __launch_bounds__(256) void atomic_rate_kernel(double* t, uint32_t slots, uint32_t per_lane)
+{
+    uint32_t x = (blockIdx.x * 256u + threadIdx.x) * 2654435761u + 0x9E3779B9u;
+    for (uint32_t i = 0; i < per_lane; ++i) {
+        x ^= x << 13;
+        x ^= x >> 17;
+        x ^= x << 5;
+        unsafeAtomicAdd(&t[x % slots], 1.0);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_atomic_rate(double* t, long slots, long count, hipStream_t s)
+{
+    // 2048 blocks of 256 lanes; per_lane adds each
+    const long lanes = 2048L * 256L;
+    long per = count / lanes;
+    if (per < 1) per = 1;
+    if (slots < 1 || slots > 0xffffffffL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(atomic_rate_kernel, dim3(2048), dim3(256), 0, s, t, (uint32_t)slots, (uint32_t)per);
+    return hipGetLastError();
+}
 
 hipError_t launch_mc(const McArgs& a, hipStream_t s)
 {

@@ -50,7 +50,7 @@ struct McArgs {
 
 // counters[TAM_CNT_N] is the transport's packet queue head
 enum { TAM_CNT_ABSORBED = 0, TAM_CNT_SPECULAR, TAM_CNT_ESC0, TAM_CNT_MISSED = TAM_CNT_ESC0 + 6,
-       TAM_CNT_CROSS, TAM_CNT_INTER, TAM_CNT_KILLED, TAM_CNT_N, TAM_CNT_SLOTS };
+       TAM_CNT_CROSS, TAM_CNT_INTER, TAM_CNT_KILLED, TAM_CNT_TALLY, TAM_CNT_N, TAM_CNT_SLOTS };
 
 struct HeatArgs {
     int nx, ny, nz;
@@ -187,6 +187,8 @@ struct tam_ctx {
 
 // ---- launchers (each enqueues on `s`, returns hipError_t) --------------
 hipError_t launch_mc(const McArgs& a, hipStream_t s);
+// `count` FP64 atomic adds (whole waves) to pseudo-random slots of t[0, slots)
+hipError_t launch_atomic_rate(double* t, long slots, long count, hipStream_t s);
 hipError_t launch_heat_step(const HeatArgs& a, const float* Tin, float* Tout, float lon, double time_after,
                             int boil, int robin, int damage, int energy, int lut, int tile, int kz,
                             hipStream_t s);

@@ -17,7 +17,7 @@ class CycleRunner:
         self.tuned_mc = self.grid.tune_mc(min(int(prob.packets), 5_000_000)) if tune else None
         self.time = 0.0
         self.cycle = 0
-        self.totals = dict(packets=0.0, crossings=0.0, interactions=0.0, mc_ms=0.0, heat_ms=0.0,
+        self.totals = dict(packets=0.0, crossings=0.0, interactions=0.0, tally_adds=0.0, mc_ms=0.0, heat_ms=0.0,
                            heat_launches=0, rebuild_ms=0.0, normalise_ms=0.0, arrhenius_ms=0.0,
                            voxel_updates=0.0)
 
@@ -51,6 +51,7 @@ class CycleRunner:
         T["packets"] += st.packets
         T["crossings"] += st.crossings
         T["interactions"] += st.interactions
+        T["tally_adds"] += st.tally_adds
         T["mc_ms"] += st.kernel_ms
         T["heat_ms"] += t.heat_ms
         T["heat_launches"] += t.heat_steps

@@ -27,7 +27,7 @@ EXPORTED = ("tam_abi_version", "tam_create", "tam_destroy", "tam_last_error", "t
             "tam_set_thermal", "tam_mc_run", "tam_normalise", "tam_heat_run", "tam_arrhenius",
             "tam_rebuild", "tam_download", "tam_upload", "tam_energy", "tam_get_timing",
             "tam_synchronize", "tam_set_heat_launch", "tam_tune_heat", "tam_set_mc_launch", "tam_tune_mc",
-            "tam_copy_bandwidth", "tam_struct_size", "tam_pulse_advance")
+            "tam_copy_bandwidth", "tam_atomic_rate", "tam_struct_size", "tam_pulse_advance")
 
 
 class Params(C.Structure):
@@ -57,7 +57,8 @@ class Material(C.Structure):
 class MCStats(C.Structure):
     _fields_ = [("packets", C.c_double), ("absorbed", C.c_double), ("specular", C.c_double),
                 ("escaped", C.c_double * 6), ("missed", C.c_double), ("crossings", C.c_double),
-                ("interactions", C.c_double), ("kernel_ms", C.c_double), ("killed", C.c_double)]
+                ("interactions", C.c_double), ("kernel_ms", C.c_double), ("killed", C.c_double),
+                ("tally_adds", C.c_double)]
 
     def fractions(self):
         n = self.packets
@@ -122,6 +123,7 @@ def load():
     L.tam_set_heat_launch.argtypes = [vp, C.c_int32, C.c_int32]
     L.tam_set_mc_launch.argtypes = [vp, C.c_int32, C.c_int32]
     L.tam_copy_bandwidth.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
+    L.tam_atomic_rate.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(C.c_double)]
     L.tam_struct_size.argtypes = [C.c_int32]
     L.tam_struct_size.restype = C.c_int64
     L.tam_pulse_advance.argtypes = [C.POINTER(Pulse), C.c_double]
@@ -246,6 +248,12 @@ class TissueGrid:
         g = C.c_double()
         self._chk(self.L.tam_copy_bandwidth(self.h, int(nbytes), int(reps), C.byref(g)))
         return g.value
+
+    def atomic_rate(self, count=1 << 28, slots=8 << 20):
+        """Measured FP64 atomic adds per second (random slots, no return)."""
+        r = C.c_double()
+        self._chk(self.L.tam_atomic_rate(self.h, int(count), int(slots), C.byref(r)))
+        return r.value
 
     def set_mc_launch(self, blocks_per_cu, rng_batch=32):
         self._chk(self.L.tam_set_mc_launch(self.h, int(blocks_per_cu), int(rng_batch)))
