@@ -126,6 +126,7 @@ typedef struct {
     double heat_step_ms;         /* mean device time of one stencil launch in the last tam_heat_run */
     double class_mode;           /* 1: stencil runs on thermal classes + LDS coefficient table */
     double tile, kz;             /* stencil launch shape in use */
+    double cycle_ms;             /* device time of the last timed tam_cycle (transfer + graph) */
 } tam_timing;
 
 /* --- lifetime -------------------------------------------------------- */
@@ -204,6 +205,25 @@ int tam_arrhenius(tam_ctx* ctx, double delt, double time);
 /* --- setupThermalCoeff (3dFD.f90:312-361): ablation mask, property and
  * optics rebuild ("ch_opt rebuild"), then the stencil coefficients. */
 int tam_rebuild(tam_ctx* ctx);
+
+/* --- one whole coupling cycle: the body of mcpolar's time loop
+ * (mcpolar.f90:148-186) as one replay of a HIP graph.  In order:
+ *   transport of npackets (0: the laser is off, no transport, jmean keeps
+ *   its last value as the reference's jmeanGLOBAL does) on Philox stream
+ *   `stream` of key `seed`; normalisation with `scale`; `loops` heat steps
+ *   with the pulse state advanced per step as in tam_heat_run; Arrhenius
+ *   at the reference cadence with (arr_delt, time after the steps) when
+ *   arr_delt > 0; the rebuild of tam_rebuild.
+ * The graph is captured at the first call and replayed while loops, the
+ * packet capacity, seed and cadence stay the same (any tam_set_* /
+ * tam_upload / launch-shape call drops it).  Per-cycle values go to the
+ * device through a pinned staging ring, so with stats == NULL the call
+ * returns without waiting for the GPU.  With stats it waits, reports the
+ * transport counters summed since the previous timed call, checks for
+ * negative temperatures (TAM_ERR_NEGATIVE_TEMP) and sets
+ * tam_timing.cycle_ms. */
+int tam_cycle(tam_ctx* ctx, int64_t npackets, uint64_t seed, uint32_t stream, double scale, int32_t loops,
+              double arr_delt, tam_pulse* pulse, tam_mc_stats* stats);
 
 /* --- data movement ---------------------------------------------------- */
 int tam_download(tam_ctx* ctx, int32_t field, double* out);

@@ -59,6 +59,28 @@ struct Dev {   // make the context's device current for the scope of a call
     }
 };
 
+McArgs mc_args(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream)
+{
+    McArgs a{};
+    a.nx = c->p.nx; a.ny = c->p.ny; a.nz = c->p.nz;
+    a.dx = (float)(2.0 * c->p.xmax / c->p.nx);
+    a.dy = (float)(2.0 * c->p.ymax / c->p.ny);
+    a.dz = (float)(2.0 * c->p.zmax / c->p.nz);
+    a.cx = (float)c->p.xmax;
+    a.cy = (float)c->p.ymax;
+    a.radius = (float)c->p.beam_radius;
+    a.beam = c->p.beam;
+    a.n_zlo = (float)c->p.n_ext_zlo;
+    a.n_zhi = (float)c->p.n_ext_zhi;
+    a.mut = c->mut; a.mat = c->mat; a.mtab = c->mtab;
+    a.tally = c->tally; a.counters = c->counters;
+    a.p0 = 0; a.npk = (uint64_t)npackets;
+    a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32); a.stream = stream;
+    a.blocks_per_cu = c->mc_bpc;
+    a.rng_batch = c->mc_batch;
+    return a;
+}
+
 HeatArgs heat_args(tam_ctx* c)
 {
     HeatArgs a{};
@@ -269,6 +291,102 @@ int refresh_src(tam_ctx* c)
     return TAM_OK;
 }
 
+// ---- captured coupling cycle -------------------------------------------
+
+// Drop the captured graphs (state or launch shape changed; the next
+// tam_cycle captures again).
+void cycle_reset(tam_ctx* c)
+{
+    if ((c->cyc_exec[0] || c->cyc_exec[1]) && c->stream) hipStreamSynchronize(c->stream);
+    for (int p = 0; p < 2; ++p)
+        if (c->cyc_exec[p]) {
+            hipGraphExecDestroy(c->cyc_exec[p]);
+            c->cyc_exec[p] = nullptr;
+        }
+    c->cyc_loops = -1;
+}
+
+// parameter block: header, float lon[loops] (padded to 8 bytes), double t_after[loops]
+size_t cycle_lon_off() { return sizeof(CycleHdr); }
+size_t cycle_ta_off(int loops) { return sizeof(CycleHdr) + (((size_t)loops * 4 + 7) & ~(size_t)7); }
+size_t cycle_bytes(int loops) { return cycle_ta_off(loops) + (size_t)loops * 8; }
+
+// One coupling cycle on c->stream, reading the per-cycle values from the
+// device parameter block.  Swaps the host's ping-pong pointers as the steps
+// are enqueued (the caller restores them after a capture).
+int enqueue_cycle(tam_ctx* c)
+{
+    const CycleHdr* hdr = (const CycleHdr*)c->cyc_dev;
+    const float* lon = (const float*)(c->cyc_dev + cycle_lon_off());
+    const double* ta = (const double*)(c->cyc_dev + cycle_ta_off(c->cyc_loops));
+    hipStream_t s = c->stream;
+    // transport (mcpolar.f90:151-170) and normalisation (:173-174)
+    HIPCHK(c, launch_fill_u64(c->counters, TAM_CNT_SLOTS, s));
+    HIPCHK(c, launch_fill_f64(c->tally, c->nvox, 0.0, s));
+    McArgs m = mc_args(c, c->cyc_npk, c->cyc_seed, 0);
+    m.dyn_npk = &hdr->npk;
+    m.dyn_stream = &hdr->stream;
+    HIPCHK(c, launch_mc(m, s));
+    HIPCHK(c, launch_acc_u64(c->cnt_acc, c->counters, TAM_CNT_N, s));
+    HIPCHK(c, launch_normalise(c->p.nx, c->p.ny, c->p.nz, c->px, c->pxy, c->tally, c->rhokap, c->mat,
+                               c->mua_frac, hdr, 0.0, c->coeff, c->jmean, c->src, s));
+    // heat_sim_3D (3dFD.f90:21-230)
+    const int boil = c->p.boil != 0, robin = c->p.robin_top != 0, dmg = c->p.damage_every_step != 0;
+    HeatArgs a = heat_args(c);
+    Heat64Args a64 = heat64_args(c);
+    for (int k = 0; k < c->cyc_loops; ++k) {
+        if (c->fp64) {
+            a64.dyn_lon = lon + k;
+            a64.dyn_tafter = ta + k;
+            HIPCHK(c, launch_heat64_step(a64, c->cur64, c->nxt64, 0.0, 0.0, boil, robin, dmg, 1, s));
+            std::swap(c->cur64, c->nxt64);
+        } else {
+            a.dyn_lon = lon + k;
+            a.dyn_tafter = ta + k;
+            HIPCHK(c, launch_heat_step(a, c->cur, c->nxt, 0.f, 0.0, boil, robin, dmg, 1, c->lut ? 1 : 0, c->tile,
+                                       c->kz, s));
+            std::swap(c->cur, c->nxt);
+        }
+    }
+    // Arrhenius at the reference cadence (mcpolar.f90:181)
+    if (c->cyc_arr) {
+        HeatArgs aa = heat_args(c);
+        aa.dyn_arr = hdr->arr;
+        HIPCHK(c, launch_arrhenius(aa, c->cur, c->fp64 ? c->cur64 : nullptr, 0.0, 0.0, s));
+    }
+    // setupThermalCoeff (mcpolar.f90:183, 3dFD.f90:312-361)
+    const RebuildArgs r = rebuild_args(c);
+    if (c->p.thermal_mode == TAM_THERMAL_WATER) HIPCHK(c, launch_water_update(r, s));
+    HIPCHK(c, launch_removal_mask(r, s));
+    HIPCHK(c, launch_apply_props(r, s));
+    HIPCHK(c, build_coeffs(c));
+    return refresh_src(c);
+}
+
+// Capture enqueue_cycle from the current ping-pong parity into cyc_exec[par].
+int capture_cycle(tam_ctx* c, int par)
+{
+    float* cur = c->cur; float* nxt = c->nxt;
+    double* cur64 = c->cur64; double* nxt64 = c->nxt64;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    const int rc = enqueue_cycle(c);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->cur = cur; c->nxt = nxt; c->cur64 = cur64; c->nxt64 = nxt64;
+    if (rc != TAM_OK) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(c, "hipStreamEndCapture", e);
+    const hipError_t ei = hipGraphInstantiate(&c->cyc_exec[par], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        c->cyc_exec[par] = nullptr;
+        return hip_fail(c, "hipGraphInstantiate", ei);
+    }
+    return TAM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -366,6 +484,7 @@ int tam_create(const tam_params* p, tam_ctx** out)
     c->kz = 16;
     c->mc_bpc = 8;
     c->mc_batch = 32;
+    c->cyc_loops = -1;
     // every stencil tile is at least 32 x 4 and kz >= 1
     c->neblk = ((nx + 31) / 32) * ((ny + 3) / 4) * nz;
     if (e == hipSuccess) e = dalloc(&c->eblk, c->neblk);
@@ -439,6 +558,12 @@ void tam_destroy(tam_ctx* c)
                     c->robin_lut, c->eblk};
     for (void* q : ptrs)
         if (q) hipFree(q);
+    cycle_reset(c);
+    if (c->cyc_dev) hipFree(c->cyc_dev);
+    if (c->cyc_host) hipHostFree(c->cyc_host);
+    if (c->cnt_acc) hipFree(c->cnt_acc);
+    for (int i = 0; i < 4; ++i)
+        if (c->cyc_ev[i]) hipEventDestroy(c->cyc_ev[i]);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -449,6 +574,7 @@ int tam_set_optics(tam_ctx* c, const uint8_t* mat, const tam_material* table, in
                    const double* rhokap)
 {
     if (!c) return TAM_ERR_ARG;
+    cycle_reset(c);                 // captured cycles hold the old state / shapes
     if (!mat || !table || nmat < 1 || nmat > TAM_MAX_MATERIALS - 1)
         return fail(c, TAM_ERR_ARG, "need 1..254 materials and a material grid");
     Dev d(c->dev);
@@ -524,6 +650,7 @@ int tam_set_thermal(tam_ctx* c, const double* temp, const double* kappa, const d
                     const double* heatcap, const double* Q, const double* W)
 {
     if (!c) return TAM_ERR_ARG;
+    cycle_reset(c);                 // captured cycles hold the old state / shapes
     if (!temp || !kappa || !density || !heatcap) return fail(c, TAM_ERR_ARG, "null thermal array");
     Dev d(c->dev);
     const long H = c->nhalo, V = c->nvox;
@@ -585,23 +712,7 @@ int tam_mc_run(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream, int
     Dev d(c->dev);
     if (!accumulate) HIPCHK(c, launch_fill_f64(c->tally, c->nvox, 0.0, c->stream));
     HIPCHK(c, launch_fill_u64(c->counters, TAM_CNT_SLOTS, c->stream));
-    McArgs a{};
-    a.nx = c->p.nx; a.ny = c->p.ny; a.nz = c->p.nz;
-    a.dx = (float)(2.0 * c->p.xmax / c->p.nx);
-    a.dy = (float)(2.0 * c->p.ymax / c->p.ny);
-    a.dz = (float)(2.0 * c->p.zmax / c->p.nz);
-    a.cx = (float)c->p.xmax;
-    a.cy = (float)c->p.ymax;
-    a.radius = (float)c->p.beam_radius;
-    a.beam = c->p.beam;
-    a.n_zlo = (float)c->p.n_ext_zlo;
-    a.n_zhi = (float)c->p.n_ext_zhi;
-    a.mut = c->mut; a.mat = c->mat; a.mtab = c->mtab;
-    a.tally = c->tally; a.counters = c->counters;
-    a.p0 = 0; a.npk = (uint64_t)npackets;
-    a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32); a.stream = stream;
-    a.blocks_per_cu = c->mc_bpc;
-    a.rng_batch = c->mc_batch;
+    const McArgs a = mc_args(c, npackets, seed, stream);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, launch_mc(a, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -633,7 +744,7 @@ int tam_normalise(tam_ctx* c, double scale)
     Dev d(c->dev);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, launch_normalise(c->p.nx, c->p.ny, c->p.nz, c->px, c->pxy, c->tally, c->rhokap, c->mat,
-                               c->mua_frac, 1, scale, c->coeff, c->jmean, c->src, c->stream));
+                               c->mua_frac, nullptr, scale, c->coeff, c->jmean, c->src, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->timing.normalise_ms = elapsed(c);
@@ -703,6 +814,106 @@ int tam_rebuild(tam_ctx* c)
     return TAM_OK;
 }
 
+int tam_cycle(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream, double scale, int32_t loops,
+              double arr_delt, tam_pulse* pulse, tam_mc_stats* stats)
+{
+    if (!c) return TAM_ERR_ARG;
+    if (npackets < 0 || loops < 0) return fail(c, TAM_ERR_ARG, "negative packet or step count");
+    if (!c->optics_set || !c->thermal_set) return fail(c, TAM_ERR_STATE, "optics and thermal state must be set");
+    Dev d(c->dev);
+    // capture key: a different step count, packet capacity, seed or damage
+    // cadence needs a new graph (npackets == 0 reuses the current one)
+    const int arr = arr_delt > 0.0 ? 1 : 0;
+    const long cap = npackets > 0 ? (long)npackets : (c->cyc_loops >= 0 && c->cyc_npk > 0 ? c->cyc_npk : 1);
+    if (loops != c->cyc_loops || cap != c->cyc_npk || seed != c->cyc_seed || arr != c->cyc_arr) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        cycle_reset(c);
+        const size_t need = cycle_bytes(loops);
+        if (need > c->cyc_bytes) {
+            if (c->cyc_dev) HIPCHK(c, hipFree(c->cyc_dev));
+            if (c->cyc_host) HIPCHK(c, hipHostFree(c->cyc_host));
+            c->cyc_dev = nullptr;
+            c->cyc_host = nullptr;
+            c->cyc_bytes = 0;
+            HIPCHK(c, hipMalloc((void**)&c->cyc_dev, need));
+            HIPCHK(c, hipHostMalloc((void**)&c->cyc_host, 4 * need, hipHostMallocDefault));
+            c->cyc_bytes = need;
+            for (int i = 0; i < 4; ++i) c->cyc_ev_used[i] = false;
+        }
+        if (!c->cnt_acc) {
+            HIPCHK(c, dalloc(&c->cnt_acc, TAM_CNT_N));
+            HIPCHK(c, launch_fill_u64(c->cnt_acc, TAM_CNT_N, c->stream));
+        }
+        for (int i = 0; i < 4; ++i)
+            if (!c->cyc_ev[i]) HIPCHK(c, hipEventCreateWithFlags(&c->cyc_ev[i], hipEventDisableTiming));
+        c->cyc_loops = loops;
+        c->cyc_npk = cap;
+        c->cyc_seed = seed;
+        c->cyc_arr = arr;
+    }
+    if (npackets > c->cyc_npk) return fail(c, TAM_ERR_ARG, "packet count above the captured capacity");
+
+    // ---- this cycle's parameter block in a free staging slot
+    const int slot = c->cyc_slot;
+    if (c->cyc_ev_used[slot]) HIPCHK(c, hipEventSynchronize(c->cyc_ev[slot]));
+    char* hb = c->cyc_host + (size_t)slot * c->cyc_bytes;
+    CycleHdr* h = (CycleHdr*)hb;
+    std::memset(h, 0, sizeof(CycleHdr));
+    h->npk = npackets;
+    h->stream = stream;
+    h->mc_on = npackets > 0 ? 1 : 0;
+    h->scale = scale;
+    float* lon = (float*)(hb + cycle_lon_off());
+    double* ta = (double*)(hb + cycle_ta_off(loops));
+    for (int k = 0; k < loops; ++k) {             // the host side of heat_sim_3D's step loop
+        lon[k] = (float)((pulse && pulse->use_pulse) ? pulse->laserOn : 1.0);
+        ta[k] = (pulse ? pulse->time : c->time) + c->p.delt;
+        if (pulse) tam_pulse_advance(pulse, c->p.delt);
+        else c->time += c->p.delt;
+    }
+    h->arr[0] = arr_delt;
+    h->arr[1] = pulse ? pulse->time : c->time;
+
+    // ---- graph for the current ping-pong parity (captured on first use)
+    const int par = c->fp64 ? (c->cur64 == c->T64a ? 0 : 1) : (c->cur == c->dT0 ? 0 : 1);
+    if (!c->cyc_exec[par]) {
+        const int rc = capture_cycle(c, par);
+        if (rc != TAM_OK) return rc;
+    }
+    const bool timed = stats != nullptr;
+    if (timed) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->cyc_dev, hb, cycle_bytes(loops), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(c->cyc_ev[slot], c->stream));
+    c->cyc_ev_used[slot] = true;
+    c->cyc_slot = (slot + 1) % 4;
+    HIPCHK(c, hipGraphLaunch(c->cyc_exec[par], c->stream));
+    if (timed) HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    if (loops & 1) {
+        std::swap(c->cur, c->nxt);
+        std::swap(c->cur64, c->nxt64);
+    }
+    if (!timed) return TAM_OK;
+
+    // ---- synchronous form: this call's (and earlier untimed calls') counters
+    unsigned long long cnt[TAM_CNT_N];
+    HIPCHK(c, hipMemcpyAsync(cnt, c->cnt_acc, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, launch_fill_u64(c->cnt_acc, TAM_CNT_N, c->stream));
+    const int rc = check_flags(c);                 // synchronizes
+    c->timing.cycle_ms = elapsed(c);
+    std::memset(stats, 0, sizeof(*stats));
+    stats->absorbed = (double)cnt[TAM_CNT_ABSORBED];
+    stats->specular = (double)cnt[TAM_CNT_SPECULAR];
+    for (int f = 0; f < 6; ++f) stats->escaped[f] = (double)cnt[TAM_CNT_ESC0 + f];
+    stats->missed = (double)cnt[TAM_CNT_MISSED];
+    stats->crossings = (double)cnt[TAM_CNT_CROSS];
+    stats->interactions = (double)cnt[TAM_CNT_INTER];
+    stats->killed = (double)cnt[TAM_CNT_KILLED];
+    stats->tally_adds = (double)cnt[TAM_CNT_TALLY];
+    stats->packets = stats->absorbed + stats->specular + stats->missed + stats->killed;
+    for (int f = 0; f < 6; ++f) stats->packets += stats->escaped[f];
+    return rc;
+}
+
 int tam_download(tam_ctx* c, int32_t field, double* out)
 {
     if (!c) return TAM_ERR_ARG;
@@ -752,6 +963,7 @@ int tam_download(tam_ctx* c, int32_t field, double* out)
 int tam_upload(tam_ctx* c, int32_t field, const double* in)
 {
     if (!c) return TAM_ERR_ARG;
+    cycle_reset(c);                 // captured cycles hold the old state / shapes
     if (!in) return fail(c, TAM_ERR_ARG, "null input");
     Dev d(c->dev);
     const long H = c->nhalo, V = c->nvox;
@@ -888,6 +1100,7 @@ int tam_atomic_rate(tam_ctx* c, int64_t count, int64_t slots, double* per_s)
 int tam_set_mc_launch(tam_ctx* c, int32_t blocks_per_cu, int32_t rng_batch)
 {
     if (!c) return TAM_ERR_ARG;
+    cycle_reset(c);                 // captured cycles hold the old state / shapes
     if (blocks_per_cu < 1 || blocks_per_cu > 64) return fail(c, TAM_ERR_ARG, "blocks_per_cu must be in [1, 64]");
     if (rng_batch < 1 || rng_batch > 64) return fail(c, TAM_ERR_ARG, "rng_batch must be in [1, 64]");
     c->mc_bpc = blocks_per_cu;
@@ -899,6 +1112,7 @@ int tam_tune_mc(tam_ctx* c, int64_t sample, int32_t* blocks_per_cu, int32_t* rng
 {
     if (!c) return TAM_ERR_ARG;
     if (sample < 1) return fail(c, TAM_ERR_ARG, "sample must be positive");
+    const int old_bpc = c->mc_bpc, old_batch = c->mc_batch;
     const int bpcs[] = {2, 4, 6, 8};
     const int batches[] = {8, 16, 24, 32, 40, 48};
     double best = 0.0;
@@ -916,6 +1130,7 @@ int tam_tune_mc(tam_ctx* c, int64_t sample, int32_t* blocks_per_cu, int32_t* rng
     }
     c->mc_bpc = bb;
     c->mc_batch = bt;
+    if (bb != old_bpc || bt != old_batch) cycle_reset(c);   // captured cycles use the old shape
     Dev d(c->dev);
     HIPCHK(c, launch_fill_f64(c->tally, c->nvox, 0.0, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -928,6 +1143,7 @@ int tam_tune_mc(tam_ctx* c, int64_t sample, int32_t* blocks_per_cu, int32_t* rng
 int tam_set_heat_launch(tam_ctx* c, int32_t tile, int32_t kz)
 {
     if (!c) return TAM_ERR_ARG;
+    cycle_reset(c);                 // captured cycles hold the old state / shapes
     if (tile < 0 || tile >= heat_num_tiles() || kz < 1) return fail(c, TAM_ERR_ARG, "bad stencil launch shape");
     c->tile = tile;
     c->kz = kz;
@@ -987,6 +1203,7 @@ int tam_tune_heat(tam_ctx* c, int32_t reps, int32_t* tile, int32_t* kz, double* 
         (void)hipFree(eb);
         (void)hipFree(qs);
         if (e != hipSuccess) return hip_fail(c, "heat tuning", e);
+        if (bt != c->tile || bk != c->kz) cycle_reset(c);     // captured cycles use the old shape
         c->tile = bt;
         c->kz = bk;
     }

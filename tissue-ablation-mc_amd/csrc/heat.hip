@@ -65,6 +65,10 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
                                                          float* __restrict__ Tout, float lon, double t_after,
                                                          int kz, unsigned flags)
 {
+    if (a.dyn_lon) {              // graph replay: this step's values from device memory
+        lon = *a.dyn_lon;
+        t_after = *a.dyn_tafter;
+    }
     __shared__ float sT[TY + 2][TX + 2];
     __shared__ uint8_t sC[LUT ? TY + 2 : 1][LUT ? TX + 2 : 1];
     __shared__ float sGx[LUT ? 1 : TY][LUT ? 1 : TX + 1];
@@ -289,6 +293,10 @@ __global__ __launch_bounds__(TX* TY) void stencil_direct_kernel(const HeatArgs a
                                                                 float* __restrict__ Tout, float lon,
                                                                 double t_after, int kz, unsigned flags)
 {
+    if (a.dyn_lon) {              // graph replay: this step's values from device memory
+        lon = *a.dyn_lon;
+        t_after = *a.dyn_tafter;
+    }
     __shared__ float sLut[LUT ? 3 * 256 + 16 : 1];   // face table + heatcap*mass per class
     __shared__ float2 sRob[LUT ? 16 : 1];
     const float* __restrict__ gxa = a.gx;
@@ -427,6 +435,10 @@ __global__ __launch_bounds__(256) void arrhenius_kernel(HeatArgs a, const float*
                                                         const double* __restrict__ T64, double delt,
                                                         double time)
 {
+    if (a.dyn_arr) {              // graph replay: (delt, time) from device memory
+        delt = a.dyn_arr[0];
+        time = a.dyn_arr[1];
+    }
     const long nv = (long)a.nx * a.ny * a.nz;
     const long v = (long)blockIdx.x * 256 + threadIdx.x;
     if (v >= nv) return;

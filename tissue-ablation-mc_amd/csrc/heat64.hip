@@ -29,6 +29,10 @@ template <bool BOIL, bool ROBIN, bool DAMAGE, bool ENERGY>
 __global__ __launch_bounds__(256) void heat64_kernel(Heat64Args a, const double* __restrict__ Tin,
                                                      double* __restrict__ Tout, double lon, double time_after)
 {
+    if (a.dyn_lon) {              // graph replay: this step's values from device memory
+        lon = (double)*a.dyn_lon;
+        time_after = *a.dyn_tafter;
+    }
     const long nv = (long)a.nx * a.ny * a.nz;
     const long v = (long)blockIdx.x * 256 + threadIdx.x;
     double e_out = 0.0;

@@ -118,6 +118,8 @@ __global__ __launch_bounds__(256) void mc_kernel(McArgs a)
     const float idx_ = rcp(a.dx), idy_ = rcp(a.dy);
     const float ztop = a.dz * (float)a.nz;
     const uint32_t batch = (uint32_t)a.rng_batch;
+    const uint64_t npk = a.dyn_npk ? (uint64_t)*a.dyn_npk : a.npk;
+    const uint32_t stream = a.dyn_stream ? *a.dyn_stream : a.stream;
 
     uint32_t c_abs = 0, c_spec = 0, c_miss = 0, c_cross = 0, c_inter = 0, c_kill = 0, c_tally = 0;
     uint32_t c_esc[6] = {0, 0, 0, 0, 0, 0};
@@ -160,13 +162,13 @@ __global__ __launch_bounds__(256) void mc_kernel(McArgs a)
                 if (rank >= avail) mine = nb + (rank - avail);
                 pool = nb + (cnt - avail);
                 pool_end = nb + kChunk;
-                if (pool_end > a.npk) pool_end = a.npk > nb ? a.npk : nb;
+                if (pool_end > npk) pool_end = npk > nb ? npk : nb;
                 if (pool > pool_end) pool = pool_end;
             } else {
                 pool += cnt;
             }
             if (!have) {
-                if (mine >= a.npk) break;          // queue drained: this lane is done
+                if (mine >= npk) break;            // queue drained: this lane is done
                 have = true;
                 pend = PEND_LAUNCH;
                 pid = a.p0 + mine;
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(256) void mc_kernel(McArgs a)
         if ((uint32_t)__popcll(pm) >= batch || pm == am) {
             // ================= random phase: lanes with a pending event ====
             if (pend != PEND_NONE) {
-                const uint4 r = philox4x32_10(make_uint4((uint32_t)pid, (uint32_t)(pid >> 32), ev, a.stream),
+                const uint4 r = philox4x32_10(make_uint4((uint32_t)pid, (uint32_t)(pid >> 32), ev, stream),
                                               a.key0, a.key1);
                 ++ev;
                 const int kind = pend;

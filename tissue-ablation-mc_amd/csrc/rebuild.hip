@@ -150,12 +150,16 @@ __global__ __launch_bounds__(256) void coeff_kernel(CoeffArgs a)
 __global__ __launch_bounds__(256) void normalise_kernel(int nx, int ny, int nz, long px, long pxy,
                                                         const double* tally, const double* rhokap,
                                                         const uint8_t* mat, const double* mua_frac,
-                                                        double scale, const double* coeff,
-                                                        double* jmean, float* src)
+                                                        const CycleHdr* dyn, double scale,
+                                                        const double* coeff, double* jmean, float* src)
 {
     const long nv = (long)nx * ny * nz;
     const long v = (long)blockIdx.x * 256 + threadIdx.x;
     if (v >= nv) return;
+    if (dyn) {
+        if (!dyn->mc_on) return;
+        scale = dyn->scale;
+    }
     int i, j, k;
     vox_ijk(v, nx, ny, i, j, k);
     const long c = i + px * j + pxy * k;
@@ -198,6 +202,11 @@ __global__ void fill_u64_kernel(unsigned long long* p, long n)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0ull;
 }
+// acc += src over n counters (one block; n is a handful)
+__global__ void acc_u64_kernel(unsigned long long* acc, const unsigned long long* src, long n)
+{
+    for (long i = threadIdx.x; i < n; i += blockDim.x) acc[i] += src[i];
+}
 
 // Streaming copy (16-byte loads and stores): the measured-bandwidth
 // reference beside the HBM spec, and the calibration for byte counters.
@@ -237,12 +246,11 @@ hipError_t launch_coeffs(const CoeffArgs& a, hipStream_t s)
 }
 hipError_t launch_normalise(int nx, int ny, int nz, long px, long pxy, const double* tally,
                             const double* rhokap, const uint8_t* mat, const double* mua_frac,
-                            int use_rhokap, double scale, const double* coeff, double* jmean,
+                            const CycleHdr* dyn, double scale, const double* coeff, double* jmean,
                             float* src, hipStream_t s)
 {
-    (void)use_rhokap;
     hipLaunchKernelGGL(normalise_kernel, dim3(grid_for((long)nx * ny * nz)), dim3(256), 0, s, nx, ny, nz, px,
-                       pxy, tally, rhokap, mat, mua_frac, scale, coeff, jmean, src);
+                       pxy, tally, rhokap, mat, mua_frac, dyn, scale, coeff, jmean, src);
     return hipGetLastError();
 }
 hipError_t launch_energy(int nx, int ny, int nz, long px, long pxy, const float* dT, const double* T64,
@@ -269,6 +277,12 @@ hipError_t launch_copy(const void* in, void* out, long bytes, hipStream_t s)
     if (b > 8192) b = 8192;
     if (b < 1) b = 1;
     hipLaunchKernelGGL(copy_kernel, dim3((unsigned)b), dim3(256), 0, s, (const float4*)in, (float4*)out, n4);
+    return hipGetLastError();
+}
+
+hipError_t launch_acc_u64(unsigned long long* acc, const unsigned long long* src, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(acc_u64_kernel, dim3(1), dim3(64), 0, s, acc, src, n);
     return hipGetLastError();
 }
 

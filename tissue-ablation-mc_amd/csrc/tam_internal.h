@@ -46,6 +46,10 @@ struct McArgs {
     uint32_t key0, key1, stream;
     int blocks_per_cu;         // 256-lane blocks per CU (lanes = packets in flight)
     int rng_batch;             // pending lanes of a wave that start its random phase
+    // graph replays: packet count and Philox stream read from device memory
+    // at launch (nullptr: npk and stream above)
+    const int64_t* dyn_npk;
+    const uint32_t* dyn_stream;
 };
 
 // counters[TAM_CNT_N] is the transport's packet queue head
@@ -85,6 +89,11 @@ struct HeatArgs {
     const float2* robin_lut;   // [16] convective ghost (beta, gamma') per top-cell class
     // FP32 pre-tests (conservative) before the exact FP64 tests
     float dT_neg, dT_boil_test, dT_lo_test, dT_hi_test;
+    // graph replays: this step's laser state and time after the step, and the
+    // Arrhenius (delt, time), read from device memory (nullptr: by value)
+    const float* dyn_lon;
+    const double* dyn_tafter;
+    const double* dyn_arr;
 };
 
 #define TAM_NCLASS 16          // LUT classes; the last one is "removed" (no conduction)
@@ -107,6 +116,8 @@ struct Heat64Args {            // FP64 reference-order step (heat64.hip)
     double arr_A, arr_dE, arr_R, arr_Tlo, arr_Thi, thr0, thr1, thr2;
     double* energy;
     int* flags;
+    const float* dyn_lon;      // graph replays, as in HeatArgs
+    const double* dyn_tafter;
 };
 
 struct Coeff64Args {
@@ -183,7 +194,32 @@ struct tam_ctx {
     double time;                // heat time when no pulse state is passed
     tam_material host_tab[TAM_MAX_MATERIALS + 1];
     tam_timing timing;
+    // captured coupling cycle (tam_cycle): one executable graph per starting
+    // buffer of the temperature ping-pong (an odd step count swaps them)
+    hipGraphExec_t cyc_exec[2];
+    long cyc_npk;               // capture key: packet capacity, steps, launch shapes, flags
+    int cyc_loops, cyc_arr;
+    uint64_t cyc_seed;
+    char* cyc_dev;              // device parameter block of the replays
+    size_t cyc_bytes;
+    char* cyc_host;             // pinned staging ring of parameter blocks
+    int cyc_slot;
+    hipEvent_t cyc_ev[4];       // staging slot i is free once cyc_ev[i] completed
+    bool cyc_ev_used[4];
+    unsigned long long* cnt_acc;   // transport counters summed over replays
 };
+
+// parameter block of one graph replay (device layout, 64-byte header, then
+// float lon[loops] padded to 8 bytes, then double t_after[loops])
+struct CycleHdr {
+    int64_t npk;               // packets this cycle (0: laser off, no transport)
+    uint32_t stream;           // Philox stream id
+    int32_t mc_on;             // 1: run transport + normalise
+    double scale;              // normalisation scale
+    double arr[2];             // Arrhenius delt, time (reference cadence)
+    double pad[3];
+};
+static_assert(sizeof(CycleHdr) == 64, "cycle header layout");
 
 // ---- launchers (each enqueues on `s`, returns hipError_t) --------------
 hipError_t launch_mc(const McArgs& a, hipStream_t s);
@@ -197,15 +233,18 @@ const char* heat_tile_name(int tile);
 hipError_t launch_heat64_step(const Heat64Args& a, const double* Tin, double* Tout, double lon,
                               double time_after, int boil, int robin, int damage, int energy, hipStream_t s);
 hipError_t launch_coeffs64(const Coeff64Args& a, hipStream_t s);
+hipError_t launch_acc_u64(unsigned long long* acc, const unsigned long long* src, long n, hipStream_t s);
 hipError_t launch_arrhenius(const HeatArgs& a, const float* T, const double* T64, double delt, double time,
                             hipStream_t s);
 hipError_t launch_removal_mask(const RebuildArgs& a, hipStream_t s);
 hipError_t launch_apply_props(const RebuildArgs& a, hipStream_t s);
 hipError_t launch_water_update(const RebuildArgs& a, hipStream_t s);
 hipError_t launch_coeffs(const CoeffArgs& a, hipStream_t s);
+// dyn (graph replays, else nullptr): scale from dyn->scale; nothing is
+// written when dyn->mc_on == 0 (laser off: jmean keeps its last value)
 hipError_t launch_normalise(int nx, int ny, int nz, long px, long pxy, const double* tally,
                             const double* rhokap, const uint8_t* mat, const double* mua_tab,
-                            int use_rhokap, double scale, const double* coeff, double* jmean,
+                            const CycleHdr* dyn, double scale, const double* coeff, double* jmean,
                             float* src, hipStream_t s);
 hipError_t launch_fill_f64(double* p, long n, double v, hipStream_t s);
 hipError_t launch_fill_f32(float* p, long n, float v, hipStream_t s);

@@ -8,9 +8,13 @@ from . import lib
 
 
 class CycleRunner:
-    def __init__(self, prob, seed=0, tune=True):
+    """graph=True runs each cycle as one replay of the captured HIP graph
+    (tam_cycle); graph=False issues the same steps one C-ABI call at a time."""
+
+    def __init__(self, prob, seed=0, tune=True, graph=False):
         self.prob = prob
         self.seed = seed
+        self.graph = graph
         self.grid = prob.grid()
         # launch shapes picked by timing the candidates on this grid
         self.tuned = self.grid.tune_heat(reps=5) if tune else None
@@ -34,6 +38,8 @@ class CycleRunner:
             self.totals[k] = 0.0
 
     def run_cycle(self):
+        if self.graph:
+            return self._run_cycle_graph()
         p, g = self.prob, self.grid
         st = g.mc_run(p.packets, seed=self.seed, stream=self.cycle)
         g.normalise(p.scale)
@@ -58,6 +64,24 @@ class CycleRunner:
         T["rebuild_ms"] += t.rebuild_ms
         T["normalise_ms"] += t.normalise_ms
         T["arrhenius_ms"] += 0.0 if p.params.damage_every_step else t.arrhenius_ms
+        T["voxel_updates"] += float(p.steps) * p.params.nx * p.params.ny * p.params.nz
+        self.cycle += 1
+        return st
+
+    def _run_cycle_graph(self):
+        p, g = self.prob, self.grid
+        if self.tuned is not None and p.steps >= 200:
+            self.tuned = g.tune_heat(reps=3)         # keeps the graph unless the shape changes
+        arr = 0.0 if p.params.damage_every_step else p.params.delt
+        st = g.cycle(p.packets, self.seed, self.cycle, p.scale, p.steps, arr_delt=arr)
+        self.time += p.steps * p.params.delt
+        t = g.timing()
+        T = self.totals
+        T["packets"] += st.packets
+        T["crossings"] += st.crossings
+        T["interactions"] += st.interactions
+        T["tally_adds"] += st.tally_adds
+        T["cycle_ms"] = T.get("cycle_ms", 0.0) + t.cycle_ms
         T["voxel_updates"] += float(p.steps) * p.params.nx * p.params.ny * p.params.nz
         self.cycle += 1
         return st

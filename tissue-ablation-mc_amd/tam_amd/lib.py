@@ -27,7 +27,7 @@ EXPORTED = ("tam_abi_version", "tam_create", "tam_destroy", "tam_last_error", "t
             "tam_set_thermal", "tam_mc_run", "tam_normalise", "tam_heat_run", "tam_arrhenius",
             "tam_rebuild", "tam_download", "tam_upload", "tam_energy", "tam_get_timing",
             "tam_synchronize", "tam_set_heat_launch", "tam_tune_heat", "tam_set_mc_launch", "tam_tune_mc",
-            "tam_copy_bandwidth", "tam_atomic_rate", "tam_struct_size", "tam_pulse_advance")
+            "tam_copy_bandwidth", "tam_atomic_rate", "tam_struct_size", "tam_pulse_advance", "tam_cycle")
 
 
 class Params(C.Structure):
@@ -80,7 +80,8 @@ class Timing(C.Structure):
     _fields_ = [("mc_ms", C.c_double), ("normalise_ms", C.c_double), ("heat_ms", C.c_double),
                 ("arrhenius_ms", C.c_double), ("rebuild_ms", C.c_double),
                 ("heat_steps", C.c_double), ("heat_step_ms", C.c_double),
-                ("class_mode", C.c_double), ("tile", C.c_double), ("kz", C.c_double)]
+                ("class_mode", C.c_double), ("tile", C.c_double), ("kz", C.c_double),
+                ("cycle_ms", C.c_double)]
 
 
 class TamError(RuntimeError):
@@ -129,6 +130,8 @@ def load():
     L.tam_pulse_advance.argtypes = [C.POINTER(Pulse), C.c_double]
     L.tam_pulse_advance.restype = None
     L.tam_tune_mc.argtypes = [vp, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+    L.tam_cycle.argtypes = [vp, C.c_int64, C.c_uint64, C.c_uint32, C.c_double, C.c_int32, C.c_double,
+                            C.POINTER(Pulse), C.POINTER(MCStats)]
     L.tam_tune_heat.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                 C.POINTER(C.c_double)]
     _lib = L
@@ -192,6 +195,16 @@ class TissueGrid:
         st = MCStats()
         self._chk(self.L.tam_mc_run(self.h, int(npackets), int(seed), int(stream), int(bool(accumulate)),
                                     C.byref(st)))
+        return st
+
+    def cycle(self, npackets, seed, stream, scale, loops, arr_delt=0.0, pulse=None, stats=True):
+        """One coupling cycle as a graph replay (transport, normalise, `loops`
+        heat steps, optional Arrhenius, rebuild).  Returns the transport
+        counters when stats is true (the call then waits for the GPU)."""
+        st = MCStats() if stats else None
+        self._chk(self.L.tam_cycle(self.h, int(npackets), int(seed), int(stream), float(scale), int(loops),
+                                   float(arr_delt), None if pulse is None else C.byref(pulse),
+                                   None if st is None else C.byref(st)))
         return st
 
     def normalise(self, scale):
