@@ -243,6 +243,9 @@ int tam_copy_bandwidth(tam_ctx* ctx, int64_t bytes, int32_t reps, double* gbps);
  * 64); *per_s = adds / kernel time.  The measured ceiling of the transport's
  * tally (no reference counterpart). */
 int tam_atomic_rate(tam_ctx* ctx, int64_t count, int64_t slots, double* per_s);
+/* Wait for the context's stream; TAM_ERR_NEGATIVE_TEMP if any heat step
+ * since tam_set_thermal saw T < 0 (the reference's mpi_abort,
+ * 3dFD.f90:179-182), which asynchronous tam_cycle calls cannot report. */
 int tam_synchronize(tam_ctx* ctx);
 
 #ifdef __cplusplus

@@ -44,13 +44,14 @@ def build_lib():
 
 
 def build_fortran():
-    """Fortran drop-in host (mcgrid) over the C-ABI, if flang is present."""
+    """Fortran drop-in host (mcpolar_gpu, abi_check) over the C-ABI, if flang
+    is present; always rebuilt (-B) against the library just built."""
     fdir = os.path.join(PKG, "fortran")
     mk = os.path.join(fdir, "Makefile")
     if not (os.path.exists(FLANG) and os.path.exists(mk)):
         return None
-    run(["make", "-s", "-C", fdir, f"FLANG={FLANG}"])
-    return os.path.join(fdir, "mcgrid")
+    run(["make", "-B", "-s", "-C", fdir, f"FLANG={FLANG}"])
+    return os.path.join(fdir, "mcpolar_gpu")
 
 
 def main(argv):

@@ -1217,8 +1217,7 @@ int tam_synchronize(tam_ctx* c)
 {
     if (!c) return TAM_ERR_ARG;
     Dev d(c->dev);
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return TAM_OK;
+    return check_flags(c);           // waits for the stream, then reports T < 0 seen since
 }
 
 }  // extern "C"
