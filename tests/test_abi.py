@@ -11,22 +11,26 @@ import oracle_ffi as O
 from tam_amd import lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "tam.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("tam.h", "tam_denoise.h")]
 
 
 def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(tam_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(tam_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_function():
+    from tam_amd import denoise
     L = lib.load()
     names = declared_functions()
-    assert len(names) >= 20
+    assert len(names) >= 30
     for n in names:
         assert hasattr(L, n), n
-    assert set(names) == set(lib.EXPORTED)
+    assert set(names) == set(lib.EXPORTED) | set(denoise.EXPORTED)
+    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["tam.h", "tam_denoise.h"]
 
 
 def test_abi_version():

@@ -1,0 +1,98 @@
+"""The denoiser's BF16 MFMA kernels against the CPU reference (torch float32
+conv3d / autograd with the same BF16 rounding of stored activations).
+
+Tolerances: forward prediction within 2% of the prediction's range (BF16
+rounding of ten layers' activations can differ by one BF16 ulp where the
+FP32 sums differ in order); gradients per parameter tensor with cosine
+similarity >= 0.99 and relative L2 error <= 6% (BF16 activation gradients).
+"""
+import numpy as np
+import pytest
+
+import denoise_ref as R
+from tam_amd import configs, lib
+from tam_amd.denoise import Denoiser, param_layout
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def _features(shape, seed):
+    rng = np.random.default_rng(seed)
+    nz, ny, nx = shape
+    z = np.arange(nz)[:, None, None]
+    base = np.log1p(30.0 * np.exp(-(nz - 1 - z) / 6.0)) * np.ones(shape)
+    f0 = base + 0.3 * rng.standard_normal(shape)
+    return np.stack([np.maximum(f0, 0.0), rng.uniform(-1, 1, shape), rng.uniform(-0.3, 0.5, shape),
+                     rng.uniform(-0.5, 0.75, shape)]).astype(np.float32), base.astype(np.float32)
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (32, 64, 32)])
+def test_forward_matches_cpu_reference(shape):
+    _print(f"case: denoiser forward {shape}")
+    dn = Denoiser(shape, capacity=1, seed=1)
+    p = R.random_params(dn.n, seed=2)
+    dn.set_params(p)
+    f, _ = _features(shape, 3)
+    out = dn.forward(f)
+    ref = R.forward(R.unpack(p), f).detach().numpy()
+    dn.close()
+    span = ref.max() - ref.min()
+    err = np.abs(out - ref)
+    _print(f"range {span:.3f}, max abs err {err.max():.4f}, median {np.median(err):.2e}")
+    assert err.max() <= 0.02 * span
+    assert np.median(err) <= 2e-3 * span
+
+
+def test_gradients_match_cpu_autograd():
+    shape = (32, 32, 32)
+    _print("case: denoiser gradients 32^3")
+    dn = Denoiser(shape, capacity=1, seed=1)
+    p = R.random_params(dn.n, seed=5)
+    dn.set_params(p)
+    f, t = _features(shape, 6)
+    g, loss = dn.gradients(f, t)
+    gr, loss_r = R.gradients(p, f, t)
+    dn.close()
+    _print(f"loss gpu {loss:.6f} cpu {loss_r:.6f}")
+    assert abs(loss - loss_r) <= 1e-2 * loss_r
+    for name, off, shape_ in param_layout():
+        size = int(np.prod(shape_))
+        a, b = g[off:off + size].astype(np.float64), gr[off:off + size].astype(np.float64)
+        if name == "w0":                      # padding input channels carry no gradient
+            a = a.reshape(shape_)[:, :4].ravel()
+            b = b.reshape(shape_)[:, :4].ravel()
+        nb = np.linalg.norm(b)
+        cos = float(a @ b / (np.linalg.norm(a) * nb + 1e-30))
+        rel = float(np.linalg.norm(a - b) / (nb + 1e-30))
+        _print(f"{name:7s} |g| {nb:.3e} cos {cos:.5f} rel {rel:.3e}")
+        assert cos >= 0.99 and rel <= 0.06, name
+
+
+def test_training_lowers_the_loss_on_mc_pairs():
+    """Four MC pairs of a two-layer 32^3 tissue (2e4 vs 2e6 packets): 120
+    Adam steps lower the L1 loss, and the prediction beats the noisy input
+    on the held-out pair."""
+    n = 32
+    _print("case: denoiser training on 32^3 MC pairs")
+    dn = Denoiser((n, n, n), capacity=5, seed=3)
+    for s in range(5):
+        prob = configs.three_layer_skin(n=(n, n, n), ext=(0.5, 0.5, 0.25), packets=1, steps=1, cycles=1)
+        lo, hi = prob.grid(), prob.grid()
+        lo.mc_run(20_000, seed=100 + s)
+        hi.mc_run(2_000_000, seed=900 + s)
+        dn.store(s, lo, 20_000, hi, 2_000_000)
+        lo.close()
+        hi.close()
+    before = [dn.eval(s) for s in range(5)]
+    losses = dn.train(4, 120, lr=2e-3, seed=7)
+    after = [dn.eval(s) for s in range(5)]
+    dn.close()
+    _print("first/last losses", losses[:5], losses[-5:])
+    _print("before", before, "\nafter", after)
+    assert np.mean(losses[-20:]) < 0.8 * np.mean(losses[:5])
+    l1_pred, l1_in = after[4]
+    assert l1_pred < l1_in

@@ -1,0 +1,1033 @@
+// denoise.hip -- BF16 3D U-Net fluence denoiser: MFMA implicit-GEMM
+// convolution kernels (forward, data gradient, weight gradient), the
+// pointwise layers, the loss, Adam, and the C-ABI of tam_denoise.h.
+//
+// Layout: activations are channels-last BF16, [z][y][x][c] (x fastest over
+// voxels, c fastest overall); a 3x3x3 convolution with zero padding is the
+// GEMM  Y[m][co] = sum_k X_im2col[m][k] W[k][co],  k = tap * Cin + ci,
+// tap = 9 (dz+1) + 3 (dy+1) + (dx+1).  One v_mfma_f32_16x16x32_bf16 takes a
+// 16-voxel x 32-k slab of X (lane l: voxel l%16, k 8 (l/16) .. +7, one
+// 16-byte load of 8 consecutive channels) and a 32 x 16 slab of W (packed
+// on the host side of each update into the lane order, one 16-byte load),
+// and accumulates a 16 x 16 FP32 tile (lane l: voxels 4 (l/16) + j, output
+// channel l%16).  Cin is a multiple of 16, so 8 consecutive k never cross
+// a tap.
+//
+// The data gradient is the same kernel with the weights flipped in space and
+// transposed (W'[26 - tap][co][ci]).  The weight gradient needs k = voxels:
+// each wave stages 32 consecutive voxels of the layer input (shifted by one
+// tap, zero outside) and of the output gradient in LDS, transposed to
+// [channel][voxel], and reads 8 consecutive voxels of one channel per lane.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "tam_internal.h"
+#include "../../include/tam_denoise.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+
+__device__ __forceinline__ bf16x8 load8(const uint16_t* p) { return __builtin_bit_cast(bf16x8, *(const uint4*)p); }
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u)); }
+
+struct ConvArgs {
+    const uint16_t* x;      // input [nvox][cin]
+    const uint16_t* wp;     // packed weights [kb][nt][64][8]
+    const float* bias;      // [cout] or nullptr
+    const uint16_t* mask;   // [nvox][cout] or nullptr: output *= (mask > 0)
+    uint16_t* y;            // output [nvox][cout]
+    int D, H, W, cin, cout, kb, relu;
+};
+
+// Forward convolution (and, with flipped weights, the data gradient).
+// A wave computes MT x 16 consecutive voxels of the flattened grid (each
+// lane finds its voxel's x, y, z) for all NT x 16 output channels.
+template <int NT, int MT>
+__global__ __launch_bounds__(256) void conv_kernel(ConvArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const long nvox = (long)a.D * a.H * a.W;
+    const long mbase = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * (16 * MT);
+    if (mbase >= nvox) return;
+    int vx[MT], vy[MT], vz[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+        const long m = mbase + t * 16 + (lane & 15);
+        vx[t] = (int)(m % a.W);
+        const long r = m / a.W;
+        vy[t] = (int)(r % a.H);
+        vz[t] = (int)(r / a.H);
+    }
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[t][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    int tap = 0, ci = (lane >> 4) * 8;
+    while (ci >= a.cin) { ci -= a.cin; ++tap; }
+    const uint16_t* wl = a.wp + (long)lane * 8;
+    for (int kb = 0; kb < a.kb; ++kb) {
+        bf16x8 af[MT];
+        if (tap < 27) {
+            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+            for (int t = 0; t < MT; ++t) {
+                const int sx = vx[t] + dx, sy = vy[t] + dy, sz = vz[t] + dz;
+                const bool in = sx >= 0 && sx < a.W && sy >= 0 && sy < a.H && sz >= 0 && sz < a.D;
+                af[t] = in ? load8(a.x + (((long)sz * a.H + sy) * a.W + sx) * a.cin + ci) : zero8();
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < MT; ++t) af[t] = zero8();
+        }
+        bf16x8 bfr[NT];
+#pragma unroll
+        for (int n = 0; n < NT; ++n) bfr[n] = load8(wl + ((long)kb * NT + n) * 512);
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int n = 0; n < NT; ++n)
+                acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[t], bfr[n], acc[t][n], 0, 0, 0);
+        ci += 32;
+        while (ci >= a.cin) { ci -= a.cin; ++tap; }
+    }
+    // epilogue: lane holds voxels mbase + 16 t + 4 (lane/16) + j, channel 16 n + lane%16
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+        const int co = n * 16 + (lane & 15);
+        const float b = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long m = mbase + t * 16 + (lane >> 4) * 4 + j;
+                float v = acc[t][n][j] + b;
+                if (a.relu) v = fmaxf(v, 0.f);
+                if (a.mask && !(bf2f(a.mask[m * a.cout + co]) > 0.f)) v = 0.f;
+                a.y[m * a.cout + co] = f2bf(v);
+            }
+    }
+}
+
+struct WgradArgs {
+    const uint16_t* x;      // layer input [nvox][cin]
+    const uint16_t* dy;     // output gradient [nvox][cout]
+    float* gw;              // [27][cin][cout], accumulated with atomics
+    int D, H, W, cin, cout;
+    long blocks;            // 32-voxel segments of the flattened grid
+    long per_wg;            // segments per workgroup (a multiple of 4)
+};
+
+// Weight gradient for one tap (blockIdx.y) over a range of 32-voxel
+// segments (blockIdx.x): dW^T[co][ci] += sum_m dy[m][co] x[m + tap][ci].
+template <int CIT, int COT>
+__global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a)
+{
+    constexpr int CI = CIT * 16, CO = COT * 16;
+    constexpr int PAD = 8;                                  // row pad (elements) against bank conflicts
+    __shared__ __attribute__((aligned(16))) uint16_t sX[4][CI][32 + PAD];
+    __shared__ __attribute__((aligned(16))) uint16_t sD[4][CO][32 + PAD];
+    __shared__ float sR[CI * CO];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int tap = blockIdx.y;
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    f32x4 acc[COT][CIT];
+#pragma unroll
+    for (int i = 0; i < COT; ++i)
+#pragma unroll
+        for (int c = 0; c < CIT; ++c) acc[i][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const long b0 = (long)blockIdx.x * a.per_wg;
+    const long b1 = b0 + a.per_wg < a.blocks ? b0 + a.per_wg : a.blocks;
+    for (long bb = b0; bb < b1; bb += 4) {
+        const long blk = bb + wv;
+        const bool have = blk < b1;
+        if (have) {
+            // segment blk: the 32 consecutive voxels 32 blk .. 32 blk + 31 of
+            // the flattened grid (each finds its own x, y, z below)
+            const long mrow = blk * 32;
+            // dy: 32 voxels x CO channels, 8 channels per lane-load
+            for (int q = lane; q < 32 * CO / 8; q += 64) {
+                const int v = q / (CO / 8), c8 = (q % (CO / 8)) * 8;
+                const uint4 d = *(const uint4*)(a.dy + (mrow + v) * a.cout + c8);
+                const uint16_t* e = (const uint16_t*)&d;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) sD[wv][c8 + j][v] = e[j];
+            }
+            // x shifted by the tap, zero outside the grid
+            for (int q = lane; q < 32 * CI / 8; q += 64) {
+                const int v = q / (CI / 8), c8 = (q % (CI / 8)) * 8;
+                const long m = mrow + v;
+                const int x = (int)(m % a.W);
+                const long r = m / a.W;
+                const int sx = x + dx, sy = (int)(r % a.H) + dy, sz = (int)(r / a.H) + dz;
+                uint4 d = make_uint4(0u, 0u, 0u, 0u);
+                if (sx >= 0 && sx < a.W && sy >= 0 && sy < a.H && sz >= 0 && sz < a.D)
+                    d = *(const uint4*)(a.x + (((long)sz * a.H + sy) * a.W + sx) * a.cin + c8);
+                const uint16_t* e = (const uint16_t*)&d;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) sX[wv][c8 + j][v] = e[j];
+            }
+        }
+        __syncthreads();
+        if (have) {
+            const int k8 = (lane >> 4) * 8, r16 = lane & 15;
+            bf16x8 bx[CIT];
+#pragma unroll
+            for (int c = 0; c < CIT; ++c) bx[c] = load8(&sX[wv][c * 16 + r16][k8]);
+#pragma unroll
+            for (int i = 0; i < COT; ++i) {
+                const bf16x8 ad = load8(&sD[wv][i * 16 + r16][k8]);
+#pragma unroll
+                for (int c = 0; c < CIT; ++c)
+                    acc[i][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ad, bx[c], acc[i][c], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // reduce the four waves in LDS, then one atomic per weight
+    for (int q = threadIdx.x; q < CI * CO; q += 256) sR[q] = 0.f;
+    __syncthreads();
+    for (int w = 0; w < 4; ++w) {
+        if (wv == w) {
+#pragma unroll
+            for (int i = 0; i < COT; ++i)
+#pragma unroll
+                for (int c = 0; c < CIT; ++c)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int co = i * 16 + (lane >> 4) * 4 + j, ci = c * 16 + (lane & 15);
+                        sR[ci * CO + co] += acc[i][c][j];
+                    }
+        }
+        __syncthreads();
+    }
+    for (int q = threadIdx.x; q < CI * CO; q += 256)
+        if (sR[q] != 0.f) atomicAdd(&a.gw[(long)tap * CI * CO + q], sR[q]);
+}
+
+// bias gradient: gb[c] += sum_m d[m][c]
+__global__ __launch_bounds__(256) void bgrad_kernel(const uint16_t* d, long nvox, int c, float* gb)
+{
+    __shared__ float s[256];
+    const int ch = threadIdx.x % c;
+    const int lanes = 256 / c * c;
+    float acc = 0.f;
+    if (threadIdx.x < lanes)
+        for (long m = (long)blockIdx.x * (256 / c) + threadIdx.x / c; m < nvox; m += (long)gridDim.x * (256 / c))
+            acc += bf2f(d[m * c + ch]);
+    s[threadIdx.x] = threadIdx.x < lanes ? acc : 0.f;
+    __syncthreads();
+    if (threadIdx.x < c) {
+        float t = 0.f;
+        for (int q = threadIdx.x; q < lanes; q += c) t += s[q];
+        atomicAdd(&gb[threadIdx.x], t);
+    }
+}
+
+// 2x2x2 average pooling
+__global__ void pool_kernel(const uint16_t* x, int D, int H, int W, int C, uint16_t* y)
+{
+    const long n = (long)(D / 2) * (H / 2) * (W / 2) * C;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(q % C);
+        long r = q / C;
+        const int xo = (int)(r % (W / 2)); r /= (W / 2);
+        const int yo = (int)(r % (H / 2));
+        const int zo = (int)(r / (H / 2));
+        float s = 0.f;
+        for (int k = 0; k < 8; ++k) {
+            const int xi = 2 * xo + (k & 1), yi = 2 * yo + ((k >> 1) & 1), zi = 2 * zo + (k >> 2);
+            s += bf2f(x[(((long)zi * H + yi) * W + xi) * C + c]);
+        }
+        y[q] = f2bf(0.125f * s);
+    }
+}
+
+// cat(upsample2(coarse), skip): fine grid D x H x W, channels C1 + C2
+__global__ void upcat_kernel(const uint16_t* coarse, int C1, const uint16_t* skip, int C2, int D, int H, int W,
+                             uint16_t* y)
+{
+    const int C = C1 + C2;
+    const long n = (long)D * H * W * C;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(q % C);
+        const long m = q / C;
+        if (c < C1) {
+            const int x = (int)(m % W);
+            const long r = m / W;
+            const int yy = (int)(r % H), z = (int)(r / H);
+            y[q] = coarse[((((long)(z / 2) * (H / 2)) + yy / 2) * (W / 2) + x / 2) * C1 + c];
+        } else {
+            y[q] = skip[m * C2 + (c - C1)];
+        }
+    }
+}
+
+// gradient of the upsampled part of a concatenation, summed onto the coarse
+// grid and masked by the coarse ReLU output: d[mc][c] = (sum of the 8
+// children's g[child][c]) * (a[mc][c] > 0), c < C1
+__global__ void upcat_bwd_kernel(const uint16_t* g, int C, int C1, const uint16_t* a, int D, int H, int W,
+                                 uint16_t* d)
+{
+    const int Dc = D / 2, Hc = H / 2, Wc = W / 2;
+    const long n = (long)Dc * Hc * Wc * C1;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(q % C1);
+        long r = q / C1;
+        const int xc = (int)(r % Wc); r /= Wc;
+        const int yc = (int)(r % Hc);
+        const int zc = (int)(r / Hc);
+        float s = 0.f;
+        for (int k = 0; k < 8; ++k) {
+            const int x = 2 * xc + (k & 1), yy = 2 * yc + ((k >> 1) & 1), z = 2 * zc + (k >> 2);
+            s += bf2f(g[(((long)z * H + yy) * W + x) * C + c]);
+        }
+        d[q] = bf2f(a[q]) > 0.f ? f2bf(s) : (uint16_t)0;
+    }
+}
+
+// gradient of a ReLU output that feeds a skip connection and a pooling:
+// d[m][c] = (g_cat[m][C1 + c] + g_pool[parent(m)][c] / 8) * (a[m][c] > 0)
+__global__ void skip_pool_bwd_kernel(const uint16_t* gcat, int Ccat, int C1, const uint16_t* gpool,
+                                     const uint16_t* a, int D, int H, int W, int C, uint16_t* d)
+{
+    const long n = (long)D * H * W * C;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(q % C);
+        const long m = q / C;
+        const int x = (int)(m % W);
+        const long r = m / W;
+        const int yy = (int)(r % H), z = (int)(r / H);
+        const long mp = ((long)(z / 2) * (H / 2) + yy / 2) * (W / 2) + x / 2;
+        const float s = bf2f(gcat[m * Ccat + C1 + c]) + 0.125f * bf2f(gpool[mp * C + c]);
+        d[q] = bf2f(a[q]) > 0.f ? f2bf(s) : (uint16_t)0;
+    }
+}
+
+// features (4 x nvox FP32, or built from a context) -> 16-channel input
+__global__ void pack_input_kernel(const float* f, long nvox, uint16_t* x0)
+{
+    for (long m = (long)blockIdx.x * blockDim.x + threadIdx.x; m < nvox; m += (long)gridDim.x * blockDim.x) {
+        uint16_t v[16];
+        for (int c = 0; c < 16; ++c) v[c] = c < 4 ? f2bf(f[(long)c * nvox + m]) : (uint16_t)0;
+        uint4* o = (uint4*)(x0 + m * 16);
+        o[0] = *(uint4*)&v[0];
+        o[1] = *(uint4*)&v[8];
+    }
+}
+
+// dataset slot (4 BF16 features + BF16 target per voxel) -> 16-channel input
+__global__ void pack_slot_kernel(const uint16_t* feat, long nvox, uint16_t* x0)
+{
+    for (long m = (long)blockIdx.x * blockDim.x + threadIdx.x; m < nvox; m += (long)gridDim.x * blockDim.x) {
+        uint16_t v[16];
+        for (int c = 0; c < 16; ++c) v[c] = c < 4 ? feat[m * 4 + c] : (uint16_t)0;
+        uint4* o = (uint4*)(x0 + m * 16);
+        o[0] = *(uint4*)&v[0];
+        o[1] = *(uint4*)&v[8];
+    }
+}
+
+// features from a context: phi = L / (npk V), mu_a and mu_s from mu_t and
+// the material's albedo
+__global__ void ctx_features_kernel(const double* tally, const float* mut, const uint8_t* mat,
+                                    const MatEntry* mtab, long nvox, double inv_nv, float* f)
+{
+    for (long m = (long)blockIdx.x * blockDim.x + threadIdx.x; m < nvox; m += (long)gridDim.x * blockDim.x) {
+        const float mu = mut[m], alb = mtab[mat[m]].albedo;
+        const float mua = mu * (1.f - alb), mus = mu * alb;
+        f[m] = (float)log1p(tally[m] * inv_nv);
+        f[nvox + m] = log10f(fmaxf(mua, 1e-3f));
+        f[2 * nvox + m] = log10f(fmaxf(mus, 1e-2f)) - 2.f;
+        f[3 * nvox + m] = 5.f * (mtab[mat[m]].g - 0.8f);
+    }
+}
+
+// features (4 x nvox FP32) + target log1p(phi_hi) from a context -> slot
+__global__ void store_slot_kernel(const float* f, const double* tally_hi, double inv_nv_hi, long nvox,
+                                  uint16_t* feat, uint16_t* tgt)
+{
+    for (long m = (long)blockIdx.x * blockDim.x + threadIdx.x; m < nvox; m += (long)gridDim.x * blockDim.x) {
+        for (int c = 0; c < 4; ++c) feat[m * 4 + c] = f2bf(f[(long)c * nvox + m]);
+        tgt[m] = f2bf(tally_hi ? (float)log1p(tally_hi[m] * inv_nv_hi) : f[4 * nvox + m]);
+    }
+}
+
+// head: out[m] = x0[m][0] + hb + sum_c hw[c] a[m][c]; optional L1 against
+// a BF16 target with the loss summed into *loss
+__global__ __launch_bounds__(256) void head_kernel(const uint16_t* x0, const uint16_t* a, const float* hw,
+                                                   long nvox, float* out, const uint16_t* tgt, double* loss,
+                                                   double* base)
+{
+    __shared__ double s[2][256];
+    double l = 0.0, lb = 0.0;
+    for (long m = (long)blockIdx.x * 256 + threadIdx.x; m < nvox; m += (long)gridDim.x * 256) {
+        float v = bf2f(x0[m * 16]) + hw[16];
+        for (int c = 0; c < 16; ++c) v += hw[c] * bf2f(a[m * 16 + c]);
+        out[m] = v;
+        if (tgt) {
+            const float t = bf2f(tgt[m]);
+            l += fabsf(v - t);
+            lb += fabsf(bf2f(x0[m * 16]) - t);
+        }
+    }
+    if (!tgt) return;
+    s[0][threadIdx.x] = l;
+    s[1][threadIdx.x] = lb;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + o];
+            s[1][threadIdx.x] += s[1][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(loss, s[0][0]);
+        if (base) atomicAdd(base, s[1][0]);
+    }
+}
+
+// head backward: g = sign(out - t) / nvox; d9[m][c] = g hw[c] (a > 0);
+// ghw[c] += sum g a[m][c]; ghw[16] += sum g
+__global__ __launch_bounds__(256) void head_bwd_kernel(const float* out, const uint16_t* tgt, const uint16_t* a,
+                                                       const float* hw, long nvox, uint16_t* d9, float* ghw)
+{
+    __shared__ float s[17][256];
+    float acc[17];
+    for (int c = 0; c < 17; ++c) acc[c] = 0.f;
+    const float inv = 1.f / (float)nvox;
+    for (long m = (long)blockIdx.x * 256 + threadIdx.x; m < nvox; m += (long)gridDim.x * 256) {
+        const float diff = out[m] - bf2f(tgt[m]);
+        const float g = (diff > 0.f ? inv : (diff < 0.f ? -inv : 0.f));
+        for (int c = 0; c < 16; ++c) {
+            const float av = bf2f(a[m * 16 + c]);
+            acc[c] += g * av;
+            d9[m * 16 + c] = av > 0.f ? f2bf(g * hw[c]) : (uint16_t)0;
+        }
+        acc[16] += g;
+    }
+    for (int c = 0; c < 17; ++c) s[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int c = 0; c < 17; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 17) atomicAdd(&ghw[threadIdx.x], s[threadIdx.x][0]);
+}
+
+// Adam on the FP32 parameter vector
+__global__ void adam_kernel(float* p, const float* g, float* m1, float* m2, long n, float lr, float b1, float b2,
+                            float c1, float c2, float eps)
+{
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const float gq = g[q];
+        const float a = b1 * m1[q] + (1.f - b1) * gq;
+        const float b = b2 * m2[q] + (1.f - b2) * gq * gq;
+        m1[q] = a;
+        m2[q] = b;
+        p[q] -= lr * (a * c1) / (sqrtf(b * c2) + eps);
+    }
+}
+
+// pack FP32 W[27][cin][cout] into MFMA B fragments: [kb][nt][lane][8],
+// element (kb, nt, lane, j) = W at k = 32 kb + 8 (lane/16) + j, co = 16 nt +
+// lane%16.  flip: the data-gradient weights W'[26 - tap][co][ci] (cin and
+// cout swap roles).
+__global__ void pack_w_kernel(const float* w, int cin, int cout, int flip, int kb_n, uint16_t* wp)
+{
+    const int ci_eff = flip ? cout : cin, co_eff = flip ? cin : cout;
+    const int nt = co_eff / 16;
+    const long n = (long)kb_n * nt * 64 * 8;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int j = (int)(q % 8);
+        long r = q / 8;
+        const int lane = (int)(r % 64); r /= 64;
+        const int t = (int)(r % nt);
+        const int kb = (int)(r / nt);
+        const int k = kb * 32 + (lane >> 4) * 8 + j;
+        const int tap = k / ci_eff, ci = k % ci_eff, co = t * 16 + (lane & 15);
+        float v = 0.f;
+        if (tap < 27) v = flip ? w[((long)(26 - tap) * cin + co) * cout + ci] : w[((long)tap * cin + ci) * cout + co];
+        wp[q] = f2bf(v);
+    }
+}
+
+__global__ void fill_f32k(float* p, long n, float v)
+{
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) p[q] = v;
+}
+__global__ void fill_f64k(double* p, long n, double v)
+{
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) p[q] = v;
+}
+
+// prediction log(1 + phi) -> path length per voxel (zero where mu_t = 0)
+__global__ void write_tally_kernel(const float* out, const float* mut, long nvox, double nv, double* tally)
+{
+    for (long m = (long)blockIdx.x * blockDim.x + threadIdx.x; m < nvox; m += (long)gridDim.x * blockDim.x)
+        tally[m] = mut[m] > 0.f ? fmax(expm1((double)out[m]), 0.0) * nv : 0.0;
+}
+
+unsigned grid_of(long n, int per = 256)
+{
+    long b = (n + per - 1) / per;
+    if (b > 16384) b = 16384;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+// ---------------------------------------------------------------- network --
+
+struct Layer {
+    int cin, cout, level;       // level 0: full grid, 1: /2, 2: /4
+    long w_off, b_off;          // offsets in the parameter vector
+    int kb_f, kb_b;             // k blocks of the forward / data-gradient GEMM
+    uint16_t *wf, *wb;          // packed BF16 weights
+};
+
+constexpr int NL = 10;
+constexpr int CIN[NL] = {16, 16, 16, 32, 32, 64, 96, 32, 48, 16};
+constexpr int COUT[NL] = {16, 16, 32, 32, 64, 64, 32, 32, 16, 16};
+constexpr int LEVEL[NL] = {0, 0, 1, 1, 2, 2, 1, 1, 0, 0};
+
+}  // namespace
+
+struct tam_dn {
+    int dev;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;
+    std::string err;
+    int D[3], H[3], W[3];
+    long nv[3];
+    Layer L[NL];
+    long nparam, head_off;
+    float *p, *g, *m1, *m2;           // parameters, gradients, Adam moments
+    long step;
+    // activations (BF16) and their gradients
+    uint16_t *x0, *a[NL], *p1, *p3, *c6, *c8;
+    uint16_t *d[NL], *gc6, *gc8, *gp1, *gp3;
+    float *out, *feat;                // FP32 head output; FP32 feature staging (5 x nvox)
+    double* red;                      // [0] loss, [1] input loss
+    // dataset
+    int capacity;
+    uint16_t *ds_feat, *ds_tgt;
+    float last_ms;
+};
+
+namespace {
+
+int dfail(tam_dn* n, int code, const std::string& m)
+{
+    if (n) n->err = m;
+    return code;
+}
+#define DNCHK(n, call)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) return dfail((n), TAM_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+thread_local std::string g_dn_err;
+
+hipError_t conv_launch(tam_dn* n, const uint16_t* x, int lvl, int cin, const uint16_t* wp, int kb, const float* bias,
+                       int cout, int relu, const uint16_t* mask, uint16_t* y)
+{
+    ConvArgs a{x, wp, bias, mask, y, n->D[lvl], n->H[lvl], n->W[lvl], cin, cout, kb, relu};
+    const long nv = n->nv[lvl];
+    const int MT = 2;                                  // 32-voxel wave tiles: W >= 32
+    const unsigned blocks = (unsigned)((nv + 4 * 16 * MT - 1) / (4 * 16 * MT));
+    switch (cout) {
+    case 16: hipLaunchKernelGGL((conv_kernel<1, MT>), dim3(blocks), dim3(256), 0, n->stream, a); break;
+    case 32: hipLaunchKernelGGL((conv_kernel<2, MT>), dim3(blocks), dim3(256), 0, n->stream, a); break;
+    case 48: hipLaunchKernelGGL((conv_kernel<3, MT>), dim3(blocks), dim3(256), 0, n->stream, a); break;
+    case 64: hipLaunchKernelGGL((conv_kernel<4, MT>), dim3(blocks), dim3(256), 0, n->stream, a); break;
+    case 96: hipLaunchKernelGGL((conv_kernel<6, MT>), dim3(blocks), dim3(256), 0, n->stream, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t wgrad_launch(tam_dn* n, const uint16_t* x, const uint16_t* dy, int lvl, int cin, int cout, float* gw)
+{
+    WgradArgs a{x, dy, gw, n->D[lvl], n->H[lvl], n->W[lvl], cin, cout, 0, 0};
+    a.blocks = n->nv[lvl] / 32;
+    // about 96 workgroups per tap (27 taps: ~2600 in all)
+    long per = (a.blocks + 95) / 96;
+    per = (per + 3) / 4 * 4;
+    if (per < 4) per = 4;
+    a.per_wg = per;
+    const dim3 grid((unsigned)((a.blocks + per - 1) / per), 27);
+#define WG(CI, CO) hipLaunchKernelGGL((wgrad_kernel<CI, CO>), grid, dim3(256), 0, n->stream, a)
+    const int key = (cin / 16) * 16 + cout / 16;
+    switch (key) {
+    case 1 * 16 + 1: WG(1, 1); break;
+    case 1 * 16 + 2: WG(1, 2); break;
+    case 2 * 16 + 2: WG(2, 2); break;
+    case 2 * 16 + 4: WG(2, 4); break;
+    case 4 * 16 + 4: WG(4, 4); break;
+    case 6 * 16 + 2: WG(6, 2); break;
+    case 3 * 16 + 1: WG(3, 1); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef WG
+    return hipGetLastError();
+}
+
+hipError_t repack(tam_dn* n)
+{
+    for (int l = 0; l < NL; ++l) {
+        Layer& L = n->L[l];
+        const float* w = n->p + L.w_off;
+        hipLaunchKernelGGL(pack_w_kernel, dim3(grid_of((long)L.kb_f * L.cout * 32)), dim3(256), 0, n->stream, w,
+                           L.cin, L.cout, 0, L.kb_f, L.wf);
+        hipLaunchKernelGGL(pack_w_kernel, dim3(grid_of((long)L.kb_b * L.cin * 32)), dim3(256), 0, n->stream, w,
+                           L.cin, L.cout, 1, L.kb_b, L.wb);
+    }
+    return hipGetLastError();
+}
+
+// forward from n->x0; with tgt: the L1 loss summed into red[0] (input's in red[1])
+hipError_t forward(tam_dn* n, const uint16_t* tgt)
+{
+    hipError_t e;
+    auto conv = [&](int l, const uint16_t* in, uint16_t* o) {
+        const Layer& L = n->L[l];
+        return conv_launch(n, in, L.level, L.cin, L.wf, L.kb_f, n->p + L.b_off, L.cout, 1, nullptr, o);
+    };
+    if ((e = conv(0, n->x0, n->a[0])) != hipSuccess) return e;
+    if ((e = conv(1, n->a[0], n->a[1])) != hipSuccess) return e;
+    hipLaunchKernelGGL(pool_kernel, dim3(grid_of(n->nv[1] * 16)), dim3(256), 0, n->stream, n->a[1], n->D[0], n->H[0],
+                       n->W[0], 16, n->p1);
+    if ((e = conv(2, n->p1, n->a[2])) != hipSuccess) return e;
+    if ((e = conv(3, n->a[2], n->a[3])) != hipSuccess) return e;
+    hipLaunchKernelGGL(pool_kernel, dim3(grid_of(n->nv[2] * 32)), dim3(256), 0, n->stream, n->a[3], n->D[1], n->H[1],
+                       n->W[1], 32, n->p3);
+    if ((e = conv(4, n->p3, n->a[4])) != hipSuccess) return e;
+    if ((e = conv(5, n->a[4], n->a[5])) != hipSuccess) return e;
+    hipLaunchKernelGGL(upcat_kernel, dim3(grid_of(n->nv[1] * 96)), dim3(256), 0, n->stream, n->a[5], 64, n->a[3], 32,
+                       n->D[1], n->H[1], n->W[1], n->c6);
+    if ((e = conv(6, n->c6, n->a[6])) != hipSuccess) return e;
+    if ((e = conv(7, n->a[6], n->a[7])) != hipSuccess) return e;
+    hipLaunchKernelGGL(upcat_kernel, dim3(grid_of(n->nv[0] * 48)), dim3(256), 0, n->stream, n->a[7], 32, n->a[1], 16,
+                       n->D[0], n->H[0], n->W[0], n->c8);
+    if ((e = conv(8, n->c8, n->a[8])) != hipSuccess) return e;
+    if ((e = conv(9, n->a[8], n->a[9])) != hipSuccess) return e;
+    if (tgt) hipLaunchKernelGGL(fill_f64k, dim3(1), dim3(64), 0, n->stream, n->red, 2L, 0.0);
+    hipLaunchKernelGGL(head_kernel, dim3(grid_of(n->nv[0])), dim3(256), 0, n->stream, n->x0, n->a[9],
+                       n->p + n->head_off, n->nv[0], n->out, tgt, n->red, n->red + 1);
+    return hipGetLastError();
+}
+
+// backward after forward(tgt): gradients of the mean L1 into n->g
+hipError_t backward(tam_dn* n, const uint16_t* tgt)
+{
+    hipError_t e;
+    hipLaunchKernelGGL(fill_f32k, dim3(grid_of(n->nparam)), dim3(256), 0, n->stream, n->g, n->nparam, 0.f);
+    hipLaunchKernelGGL(head_bwd_kernel, dim3(grid_of(n->nv[0])), dim3(256), 0, n->stream, n->out, tgt, n->a[9],
+                       n->p + n->head_off, n->nv[0], n->d[9], n->g + n->head_off);
+    auto grads = [&](int l, const uint16_t* in, const uint16_t* dl) {
+        const Layer& L = n->L[l];
+        hipError_t r = wgrad_launch(n, in, dl, L.level, L.cin, L.cout, n->g + L.w_off);
+        if (r != hipSuccess) return r;
+        hipLaunchKernelGGL(bgrad_kernel, dim3(grid_of(n->nv[L.level], 256 / L.cout)), dim3(256), 0, n->stream, dl,
+                           n->nv[L.level], L.cout, n->g + L.b_off);
+        return hipGetLastError();
+    };
+    // data gradient of layer l into `o`, masked by `mask` (the ReLU output feeding l), if any
+    auto dgrad = [&](int l, const uint16_t* dl, const uint16_t* mask, uint16_t* o) {
+        const Layer& L = n->L[l];
+        return conv_launch(n, dl, L.level, L.cout, L.wb, L.kb_b, nullptr, L.cin, 0, mask, o);
+    };
+    if ((e = grads(9, n->a[8], n->d[9])) != hipSuccess) return e;
+    if ((e = dgrad(9, n->d[9], n->a[8], n->d[8])) != hipSuccess) return e;
+    if ((e = grads(8, n->c8, n->d[8])) != hipSuccess) return e;
+    if ((e = dgrad(8, n->d[8], nullptr, n->gc8)) != hipSuccess) return e;
+    hipLaunchKernelGGL(upcat_bwd_kernel, dim3(grid_of(n->nv[1] * 32)), dim3(256), 0, n->stream, n->gc8, 48, 32,
+                       n->a[7], n->D[0], n->H[0], n->W[0], n->d[7]);
+    if ((e = grads(7, n->a[6], n->d[7])) != hipSuccess) return e;
+    if ((e = dgrad(7, n->d[7], n->a[6], n->d[6])) != hipSuccess) return e;
+    if ((e = grads(6, n->c6, n->d[6])) != hipSuccess) return e;
+    if ((e = dgrad(6, n->d[6], nullptr, n->gc6)) != hipSuccess) return e;
+    hipLaunchKernelGGL(upcat_bwd_kernel, dim3(grid_of(n->nv[2] * 64)), dim3(256), 0, n->stream, n->gc6, 96, 64,
+                       n->a[5], n->D[1], n->H[1], n->W[1], n->d[5]);
+    if ((e = grads(5, n->a[4], n->d[5])) != hipSuccess) return e;
+    if ((e = dgrad(5, n->d[5], n->a[4], n->d[4])) != hipSuccess) return e;
+    if ((e = grads(4, n->p3, n->d[4])) != hipSuccess) return e;
+    if ((e = dgrad(4, n->d[4], nullptr, n->gp3)) != hipSuccess) return e;
+    hipLaunchKernelGGL(skip_pool_bwd_kernel, dim3(grid_of(n->nv[1] * 32)), dim3(256), 0, n->stream, n->gc6, 96, 64,
+                       n->gp3, n->a[3], n->D[1], n->H[1], n->W[1], 32, n->d[3]);
+    if ((e = grads(3, n->a[2], n->d[3])) != hipSuccess) return e;
+    if ((e = dgrad(3, n->d[3], n->a[2], n->d[2])) != hipSuccess) return e;
+    if ((e = grads(2, n->p1, n->d[2])) != hipSuccess) return e;
+    if ((e = dgrad(2, n->d[2], nullptr, n->gp1)) != hipSuccess) return e;
+    hipLaunchKernelGGL(skip_pool_bwd_kernel, dim3(grid_of(n->nv[0] * 16)), dim3(256), 0, n->stream, n->gc8, 48, 32,
+                       n->gp1, n->a[1], n->D[0], n->H[0], n->W[0], 16, n->d[1]);
+    if ((e = grads(1, n->a[0], n->d[1])) != hipSuccess) return e;
+    if ((e = dgrad(1, n->d[1], n->a[0], n->d[0])) != hipSuccess) return e;
+    if ((e = grads(0, n->x0, n->d[0])) != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+hipError_t adam(tam_dn* n, float lr)
+{
+    ++n->step;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f / (1.f - std::pow(b1, (float)n->step));
+    const float c2 = 1.f / (1.f - std::pow(b2, (float)n->step));
+    hipLaunchKernelGGL(adam_kernel, dim3(grid_of(n->nparam)), dim3(256), 0, n->stream, n->p, n->g, n->m1, n->m2,
+                       n->nparam, lr, b1, b2, c1, c2, 1e-8f);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return repack(n);
+}
+
+struct DevScope {
+    int prev = -1;
+    explicit DevScope(int d)
+    {
+        hipGetDevice(&prev);
+        if (prev != d) hipSetDevice(d);
+    }
+    ~DevScope()
+    {
+        int cur = -1;
+        hipGetDevice(&cur);
+        if (prev >= 0 && cur != prev) hipSetDevice(prev);
+    }
+};
+
+float ms_between(tam_dn* n)
+{
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, n->ev0, n->ev1);
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tam_dn_create(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t capacity, uint64_t seed, tam_dn** out)
+{
+    if (!out) return TAM_ERR_ARG;
+    *out = nullptr;
+    if (nx < 32 || ny < 32 || nz < 32 || nx % 32 || ny % 32 || nz % 32 || capacity < 0) {
+        g_dn_err = "grid edges must be positive multiples of 32";
+        return TAM_ERR_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_dn_err = "no HIP device";
+        return TAM_ERR_NODEVICE;
+    }
+    if (device < 0 || device >= ndev) {
+        g_dn_err = "device ordinal out of range";
+        return TAM_ERR_ARG;
+    }
+    tam_dn* n = new (std::nothrow) tam_dn();
+    if (!n) return TAM_ERR_NOMEM;
+    n->dev = device;
+    DevScope ds(device);
+    for (int l = 0; l < 3; ++l) {
+        n->W[l] = nx >> l;
+        n->H[l] = ny >> l;
+        n->D[l] = nz >> l;
+        n->nv[l] = (long)n->W[l] * n->H[l] * n->D[l];
+    }
+    long off = 0;
+    for (int l = 0; l < NL; ++l) {
+        Layer& L = n->L[l];
+        L.cin = CIN[l];
+        L.cout = COUT[l];
+        L.level = LEVEL[l];
+        L.w_off = off;
+        off += 27L * L.cin * L.cout;
+        L.b_off = off;
+        off += L.cout;
+        L.kb_f = (27 * L.cin + 31) / 32;
+        L.kb_b = (27 * L.cout + 31) / 32;
+    }
+    n->head_off = off;
+    off += 17;
+    n->nparam = off;
+    n->capacity = capacity;
+    hipError_t e = hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&n->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&n->ev1);
+    auto al = [&](void** p, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, bytes > 0 ? bytes : 16);
+    };
+    al((void**)&n->p, sizeof(float) * n->nparam);
+    al((void**)&n->g, sizeof(float) * n->nparam);
+    al((void**)&n->m1, sizeof(float) * n->nparam);
+    al((void**)&n->m2, sizeof(float) * n->nparam);
+    for (int l = 0; l < NL; ++l) {
+        Layer& L = n->L[l];
+        al((void**)&L.wf, 2ull * L.kb_f * 32 * L.cout);
+        al((void**)&L.wb, 2ull * L.kb_b * 32 * L.cin);
+        al((void**)&n->a[l], 2ull * n->nv[L.level] * L.cout);
+        al((void**)&n->d[l], 2ull * n->nv[L.level] * L.cout);
+    }
+    al((void**)&n->x0, 2ull * n->nv[0] * 16);
+    al((void**)&n->p1, 2ull * n->nv[1] * 16);
+    al((void**)&n->p3, 2ull * n->nv[2] * 32);
+    al((void**)&n->c6, 2ull * n->nv[1] * 96);
+    al((void**)&n->c8, 2ull * n->nv[0] * 48);
+    al((void**)&n->gc6, 2ull * n->nv[1] * 96);
+    al((void**)&n->gc8, 2ull * n->nv[0] * 48);
+    al((void**)&n->gp1, 2ull * n->nv[1] * 16);
+    al((void**)&n->gp3, 2ull * n->nv[2] * 32);
+    al((void**)&n->out, 4ull * n->nv[0]);
+    al((void**)&n->feat, 4ull * 5 * n->nv[0]);
+    al((void**)&n->red, 16);
+    al((void**)&n->ds_feat, 8ull * n->nv[0] * (capacity > 0 ? capacity : 1));
+    al((void**)&n->ds_tgt, 2ull * n->nv[0] * (capacity > 0 ? capacity : 1));
+    if (e != hipSuccess) {
+        g_dn_err = std::string("denoiser allocation: ") + hipGetErrorString(e);
+        tam_dn_destroy(n);
+        return e == hipErrorOutOfMemory ? TAM_ERR_NOMEM : TAM_ERR_HIP;
+    }
+    // He initialisation (input layer: 4 real channels; the 12 padding
+    // channels always see zeros), zero biases, zero head (output = input)
+    std::vector<float> h((size_t)n->nparam, 0.f);
+    std::mt19937_64 rng(seed);
+    for (int l = 0; l < NL; ++l) {
+        const Layer& L = n->L[l];
+        const int fan = 27 * (l == 0 ? 4 : L.cin);
+        std::normal_distribution<float> nd(0.f, std::sqrt(2.f / (float)fan));
+        for (int t = 0; t < 27; ++t)
+            for (int ci = 0; ci < L.cin; ++ci)
+                for (int co = 0; co < L.cout; ++co)
+                    h[(size_t)(L.w_off + ((long)t * L.cin + ci) * L.cout + co)] = (l == 0 && ci >= 4) ? 0.f : nd(rng);
+    }
+    e = hipMemcpy(n->p, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(n->m1, 0, sizeof(float) * n->nparam);
+    if (e == hipSuccess) e = hipMemset(n->m2, 0, sizeof(float) * n->nparam);
+    if (e == hipSuccess) e = hipMemset(n->x0, 0, 2ull * n->nv[0] * 16);
+    if (e == hipSuccess) e = repack(n);
+    if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
+    if (e != hipSuccess) {
+        g_dn_err = std::string("denoiser init: ") + hipGetErrorString(e);
+        tam_dn_destroy(n);
+        return TAM_ERR_HIP;
+    }
+    *out = n;
+    return TAM_OK;
+}
+
+void tam_dn_destroy(tam_dn* n)
+{
+    if (!n) return;
+    DevScope ds(n->dev);
+    if (n->stream) hipStreamSynchronize(n->stream);
+    std::vector<void*> ptrs = {n->p, n->g, n->m1, n->m2, n->x0, n->p1, n->p3, n->c6, n->c8, n->gc6, n->gc8,
+                               n->gp1, n->gp3, n->out, n->feat, n->red, n->ds_feat, n->ds_tgt};
+    for (int l = 0; l < NL; ++l) {
+        ptrs.push_back(n->L[l].wf);
+        ptrs.push_back(n->L[l].wb);
+        ptrs.push_back(n->a[l]);
+        ptrs.push_back(n->d[l]);
+    }
+    for (void* q : ptrs)
+        if (q) hipFree(q);
+    if (n->ev0) hipEventDestroy(n->ev0);
+    if (n->ev1) hipEventDestroy(n->ev1);
+    if (n->stream) hipStreamDestroy(n->stream);
+    delete n;
+}
+
+const char* tam_dn_last_error(const tam_dn* n) { return n ? n->err.c_str() : g_dn_err.c_str(); }
+
+int64_t tam_dn_num_params(const tam_dn* n) { return n ? n->nparam : -1; }
+
+int tam_dn_get_params(tam_dn* n, float* out)
+{
+    if (!n || !out) return TAM_ERR_ARG;
+    DevScope ds(n->dev);
+    DNCHK(n, hipMemcpyAsync(out, n->p, sizeof(float) * n->nparam, hipMemcpyDeviceToHost, n->stream));
+    DNCHK(n, hipStreamSynchronize(n->stream));
+    return TAM_OK;
+}
+
+int tam_dn_set_params(tam_dn* n, const float* in)
+{
+    if (!n || !in) return TAM_ERR_ARG;
+    DevScope ds(n->dev);
+    DNCHK(n, hipMemcpyAsync(n->p, in, sizeof(float) * n->nparam, hipMemcpyHostToDevice, n->stream));
+    DNCHK(n, hipMemsetAsync(n->m1, 0, sizeof(float) * n->nparam, n->stream));
+    DNCHK(n, hipMemsetAsync(n->m2, 0, sizeof(float) * n->nparam, n->stream));
+    n->step = 0;
+    DNCHK(n, repack(n));
+    DNCHK(n, hipStreamSynchronize(n->stream));
+    return TAM_OK;
+}
+
+int tam_dn_store(tam_dn* n, int32_t slot, const tam_ctx* lo, double npk_lo, const tam_ctx* hi, double npk_hi)
+{
+    if (!n || !lo || !hi) return TAM_ERR_ARG;
+    if (slot < 0 || slot >= n->capacity) return dfail(n, TAM_ERR_ARG, "slot out of range");
+    if (lo->p.nx != n->W[0] || lo->p.ny != n->H[0] || lo->p.nz != n->D[0] || hi->p.nx != n->W[0] ||
+        hi->p.ny != n->H[0] || hi->p.nz != n->D[0] || lo->dev != n->dev || hi->dev != n->dev)
+        return dfail(n, TAM_ERR_ARG, "contexts must match the network's grid and device");
+    if (!(npk_lo > 0 && npk_hi > 0)) return dfail(n, TAM_ERR_ARG, "packet counts must be positive");
+    DevScope ds(n->dev);
+    const long nv = n->nv[0];
+    const double vcm = lo->p.volume_voxel * 1e6;
+    // the contexts' streams finish their transport before this stream reads
+    DNCHK(n, hipStreamSynchronize(lo->stream));
+    DNCHK(n, hipStreamSynchronize(hi->stream));
+    hipLaunchKernelGGL(ctx_features_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream, lo->tally, lo->mut, lo->mat,
+                       lo->mtab, nv, 1.0 / (npk_lo * vcm), n->feat);
+    hipLaunchKernelGGL(store_slot_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream, n->feat, hi->tally,
+                       1.0 / (npk_hi * vcm), nv, n->ds_feat + (long)slot * nv * 4, n->ds_tgt + (long)slot * nv);
+    DNCHK(n, hipGetLastError());
+    DNCHK(n, hipStreamSynchronize(n->stream));
+    return TAM_OK;
+}
+
+int tam_dn_store_host(tam_dn* n, int32_t slot, const float* features, const float* target)
+{
+    if (!n || !features || !target) return TAM_ERR_ARG;
+    if (slot < 0 || slot >= n->capacity) return dfail(n, TAM_ERR_ARG, "slot out of range");
+    DevScope ds(n->dev);
+    const long nv = n->nv[0];
+    DNCHK(n, hipMemcpyAsync(n->feat, features, sizeof(float) * 4 * nv, hipMemcpyHostToDevice, n->stream));
+    DNCHK(n, hipMemcpyAsync(n->feat + 4 * nv, target, sizeof(float) * nv, hipMemcpyHostToDevice, n->stream));
+    hipLaunchKernelGGL(store_slot_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream, n->feat, (const double*)nullptr,
+                       0.0, nv, n->ds_feat + (long)slot * nv * 4, n->ds_tgt + (long)slot * nv);
+    DNCHK(n, hipGetLastError());
+    DNCHK(n, hipStreamSynchronize(n->stream));
+    return TAM_OK;
+}
+
+int tam_dn_train(tam_dn* n, int32_t nslots, int32_t steps, double lr, uint64_t seed, float* losses)
+{
+    if (!n) return TAM_ERR_ARG;
+    if (nslots < 1 || nslots > n->capacity || steps < 0 || !(lr > 0))
+        return dfail(n, TAM_ERR_ARG, "need 1 <= nslots <= capacity, steps >= 0, lr > 0");
+    DevScope ds(n->dev);
+    const long nv = n->nv[0];
+    std::mt19937_64 rng(seed);
+    std::vector<double> red((size_t)steps > 0 ? (size_t)steps * 2 : 2);
+    double* dred = nullptr;
+    DNCHK(n, hipMalloc((void**)&dred, sizeof(double) * 2 * (size_t)(steps > 0 ? steps : 1)));
+    DNCHK(n, hipEventRecord(n->ev0, n->stream));
+    hipError_t e = hipSuccess;
+    for (int s = 0; s < steps && e == hipSuccess; ++s) {
+        const int slot = (int)(rng() % (uint64_t)nslots);
+        const uint16_t* tgt = n->ds_tgt + (long)slot * nv;
+        hipLaunchKernelGGL(pack_slot_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream,
+                           n->ds_feat + (long)slot * nv * 4, nv, n->x0);
+        e = forward(n, tgt);
+        if (e == hipSuccess) e = hipMemcpyAsync(dred + 2 * s, n->red, 16, hipMemcpyDeviceToDevice, n->stream);
+        if (e == hipSuccess) e = backward(n, tgt);
+        if (e == hipSuccess) e = adam(n, (float)lr);
+    }
+    if (e == hipSuccess) e = hipEventRecord(n->ev1, n->stream);
+    if (e == hipSuccess && steps > 0)
+        e = hipMemcpyAsync(red.data(), dred, sizeof(double) * 2 * steps, hipMemcpyDeviceToHost, n->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
+    hipFree(dred);
+    if (e != hipSuccess) return dfail(n, TAM_ERR_HIP, std::string("training: ") + hipGetErrorString(e));
+    n->last_ms = ms_between(n);
+    if (losses)
+        for (int s = 0; s < steps; ++s) losses[s] = (float)(red[2 * (size_t)s] / (double)nv);
+    return TAM_OK;
+}
+
+int tam_dn_eval(tam_dn* n, int32_t slot, double* l1_pred, double* l1_input)
+{
+    if (!n) return TAM_ERR_ARG;
+    if (slot < 0 || slot >= n->capacity) return dfail(n, TAM_ERR_ARG, "slot out of range");
+    DevScope ds(n->dev);
+    const long nv = n->nv[0];
+    const uint16_t* tgt = n->ds_tgt + (long)slot * nv;
+    hipLaunchKernelGGL(pack_slot_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream, n->ds_feat + (long)slot * nv * 4,
+                       nv, n->x0);
+    DNCHK(n, hipEventRecord(n->ev0, n->stream));
+    DNCHK(n, forward(n, tgt));
+    DNCHK(n, hipEventRecord(n->ev1, n->stream));
+    double r[2];
+    DNCHK(n, hipMemcpyAsync(r, n->red, 16, hipMemcpyDeviceToHost, n->stream));
+    DNCHK(n, hipStreamSynchronize(n->stream));
+    n->last_ms = ms_between(n);
+    if (l1_pred) *l1_pred = r[0] / (double)nv;
+    if (l1_input) *l1_input = r[1] / (double)nv;
+    return TAM_OK;
+}
+
+int tam_dn_forward_host(tam_dn* n, const float* features, float* out)
+{
+    if (!n || !features || !out) return TAM_ERR_ARG;
+    DevScope ds(n->dev);
+    const long nv = n->nv[0];
+    DNCHK(n, hipMemcpyAsync(n->feat, features, sizeof(float) * 4 * nv, hipMemcpyHostToDevice, n->stream));
+    hipLaunchKernelGGL(pack_input_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream, n->feat, nv, n->x0);
+    DNCHK(n, hipEventRecord(n->ev0, n->stream));
+    DNCHK(n, forward(n, nullptr));
+    DNCHK(n, hipEventRecord(n->ev1, n->stream));
+    DNCHK(n, hipMemcpyAsync(out, n->out, sizeof(float) * nv, hipMemcpyDeviceToHost, n->stream));
+    DNCHK(n, hipStreamSynchronize(n->stream));
+    n->last_ms = ms_between(n);
+    return TAM_OK;
+}
+
+int tam_dn_gradients_host(tam_dn* n, const float* features, const float* target, float* grads, double* loss)
+{
+    if (!n || !features || !target || !grads) return TAM_ERR_ARG;
+    if (n->capacity < 1) return dfail(n, TAM_ERR_ARG, "needs one dataset slot");
+    DevScope ds(n->dev);
+    const long nv = n->nv[0];
+    const int rc = tam_dn_store_host(n, 0, features, target);
+    if (rc != TAM_OK) return rc;
+    const uint16_t* tgt = n->ds_tgt;
+    hipLaunchKernelGGL(pack_slot_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream, n->ds_feat, nv, n->x0);
+    DNCHK(n, forward(n, tgt));
+    DNCHK(n, backward(n, tgt));
+    double r[2];
+    DNCHK(n, hipMemcpyAsync(r, n->red, 16, hipMemcpyDeviceToHost, n->stream));
+    DNCHK(n, hipMemcpyAsync(grads, n->g, sizeof(float) * n->nparam, hipMemcpyDeviceToHost, n->stream));
+    DNCHK(n, hipStreamSynchronize(n->stream));
+    if (loss) *loss = r[0] / (double)nv;
+    return TAM_OK;
+}
+
+int tam_dn_apply(tam_dn* n, tam_ctx* c, double npk)
+{
+    if (!n || !c) return TAM_ERR_ARG;
+    if (c->p.nx != n->W[0] || c->p.ny != n->H[0] || c->p.nz != n->D[0] || c->dev != n->dev)
+        return dfail(n, TAM_ERR_ARG, "the context must match the network's grid and device");
+    if (!(npk > 0)) return dfail(n, TAM_ERR_ARG, "packet count must be positive");
+    DevScope ds(n->dev);
+    const long nv = n->nv[0];
+    const double vcm = c->p.volume_voxel * 1e6;
+    DNCHK(n, hipStreamSynchronize(c->stream));
+    DNCHK(n, hipEventRecord(n->ev0, n->stream));
+    hipLaunchKernelGGL(ctx_features_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream, c->tally, c->mut, c->mat,
+                       c->mtab, nv, 1.0 / (npk * vcm), n->feat);
+    hipLaunchKernelGGL(pack_input_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream, n->feat, nv, n->x0);
+    DNCHK(n, forward(n, nullptr));
+    hipLaunchKernelGGL(write_tally_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream, n->out, c->mut, nv, npk * vcm,
+                       c->tally);
+    DNCHK(n, hipGetLastError());
+    DNCHK(n, hipEventRecord(n->ev1, n->stream));
+    DNCHK(n, hipStreamSynchronize(n->stream));
+    n->last_ms = ms_between(n);
+    return TAM_OK;
+}
+
+double tam_dn_last_ms(const tam_dn* n) { return n ? (double)n->last_ms : 0.0; }
+
+}  // extern "C"
