@@ -33,7 +33,8 @@ extern "C" {
 typedef struct tam_dn tam_dn;
 
 /* A network for nx*ny*nz grids on `device`, with `capacity` dataset slots
- * held in device memory (10 bytes per voxel each) and He-initialised
+ * held in device memory (12 bytes per voxel each: BF16 features, FP32
+ * target) and He-initialised
  * weights drawn from `seed`. */
 int  tam_dn_create(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t capacity, uint64_t seed,
                    tam_dn** out);
@@ -61,6 +62,10 @@ int tam_dn_train(tam_dn* dn, int32_t nslots, int32_t steps, double lr, uint64_t 
 /* Mean |pred - target| and mean |input - target| (the noisy input's own
  * error) on a stored slot. */
 int tam_dn_eval(tam_dn* dn, int32_t slot, double* l1_pred, double* l1_input);
+
+/* Prediction, target and the noisy input channel (all log(1 + phi), [nvox])
+ * of a stored slot, to the host (any pointer may be NULL). */
+int tam_dn_predict_slot(tam_dn* dn, int32_t slot, float* pred, float* target, float* input);
 
 /* Host-array forms for tests: prediction log(1 + phi) [nvox] of features
  * [4][nvox]; loss and the gradient of the mean-L1 loss w.r.t. every

@@ -72,6 +72,29 @@ def test_gradients_match_cpu_autograd():
         assert cos >= 0.99 and rel <= 0.06, name
 
 
+def test_trained_weights_beat_the_noisy_input():
+    """The committed weights (trained on 2000 128^3 pairs) on a fresh layered
+    tissue they never saw: 1e5-packet input vs a 2e7-packet reference.  The
+    denoised fluence's median per-voxel error (voxels above 1% of peak) is
+    below a third of the plain low-count MC's and below 5%."""
+    import os
+    from tam_amd import learned
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tissue-ablation-mc_amd",
+                        "weights", "denoiser_c3.f32")
+    n = 128
+    _print("case: trained denoiser on a held-out 128^3 tissue")
+    dn = Denoiser((n, n, n), capacity=1, seed=0)
+    dn.set_params(np.fromfile(path, dtype=np.float32))
+    learned.make_pairs(dn, [0], n, seed=777, npk_hi=20_000_000, npk_lo_choices=(100_000,), log=_print)
+    pred, tgt, inp = dn.predict_slot(0)
+    dn.close()
+    ref = np.expm1(tgt.astype(np.float64))
+    e_in = learned.fluence_error(np.expm1(inp.astype(np.float64)), ref)
+    e_dn = learned.fluence_error(np.expm1(pred.astype(np.float64)), ref)
+    _print(f"median / p90 error: plain 1e5 {e_in[:2]}, denoised {e_dn[:2]} over {e_in[2]} voxels")
+    assert e_dn[0] < e_in[0] / 3 and e_dn[0] < 0.05
+
+
 def test_training_lowers_the_loss_on_mc_pairs():
     """Four MC pairs of a two-layer 32^3 tissue (2e4 vs 2e6 packets): 120
     Adam steps lower the L1 loss, and the prediction beats the noisy input

@@ -356,18 +356,18 @@ __global__ void ctx_features_kernel(const double* tally, const float* mut, const
 
 // features (4 x nvox FP32) + target log1p(phi_hi) from a context -> slot
 __global__ void store_slot_kernel(const float* f, const double* tally_hi, double inv_nv_hi, long nvox,
-                                  uint16_t* feat, uint16_t* tgt)
+                                  uint16_t* feat, float* tgt)
 {
     for (long m = (long)blockIdx.x * blockDim.x + threadIdx.x; m < nvox; m += (long)gridDim.x * blockDim.x) {
         for (int c = 0; c < 4; ++c) feat[m * 4 + c] = f2bf(f[(long)c * nvox + m]);
-        tgt[m] = f2bf(tally_hi ? (float)log1p(tally_hi[m] * inv_nv_hi) : f[4 * nvox + m]);
+        tgt[m] = tally_hi ? (float)log1p(tally_hi[m] * inv_nv_hi) : f[4 * nvox + m];
     }
 }
 
 // head: out[m] = x0[m][0] + hb + sum_c hw[c] a[m][c]; optional L1 against
-// a BF16 target with the loss summed into *loss
+// an FP32 target with the loss summed into *loss
 __global__ __launch_bounds__(256) void head_kernel(const uint16_t* x0, const uint16_t* a, const float* hw,
-                                                   long nvox, float* out, const uint16_t* tgt, double* loss,
+                                                   long nvox, float* out, const float* tgt, double* loss,
                                                    double* base)
 {
     __shared__ double s[2][256];
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(256) void head_kernel(const uint16_t* x0, const uin
         for (int c = 0; c < 16; ++c) v += hw[c] * bf2f(a[m * 16 + c]);
         out[m] = v;
         if (tgt) {
-            const float t = bf2f(tgt[m]);
+            const float t = tgt[m];
             l += fabsf(v - t);
             lb += fabsf(bf2f(x0[m * 16]) - t);
         }
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(256) void head_kernel(const uint16_t* x0, const uin
 
 // head backward: g = sign(out - t) / nvox; d9[m][c] = g hw[c] (a > 0);
 // ghw[c] += sum g a[m][c]; ghw[16] += sum g
-__global__ __launch_bounds__(256) void head_bwd_kernel(const float* out, const uint16_t* tgt, const uint16_t* a,
+__global__ __launch_bounds__(256) void head_bwd_kernel(const float* out, const float* tgt, const uint16_t* a,
                                                        const float* hw, long nvox, uint16_t* d9, float* ghw)
 {
     __shared__ float s[17][256];
@@ -409,7 +409,7 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const float* out, const u
     for (int c = 0; c < 17; ++c) acc[c] = 0.f;
     const float inv = 1.f / (float)nvox;
     for (long m = (long)blockIdx.x * 256 + threadIdx.x; m < nvox; m += (long)gridDim.x * 256) {
-        const float diff = out[m] - bf2f(tgt[m]);
+        const float diff = out[m] - tgt[m];
         const float g = (diff > 0.f ? inv : (diff < 0.f ? -inv : 0.f));
         for (int c = 0; c < 16; ++c) {
             const float av = bf2f(a[m * 16 + c]);
@@ -522,7 +522,8 @@ struct tam_dn {
     double* red;                      // [0] loss, [1] input loss
     // dataset
     int capacity;
-    uint16_t *ds_feat, *ds_tgt;
+    uint16_t* ds_feat;                // [slot][nvox][4] BF16 features
+    float* ds_tgt;                    // [slot][nvox] FP32 target log(1 + phi_hi)
     float last_ms;
 };
 
@@ -599,7 +600,7 @@ hipError_t repack(tam_dn* n)
 }
 
 // forward from n->x0; with tgt: the L1 loss summed into red[0] (input's in red[1])
-hipError_t forward(tam_dn* n, const uint16_t* tgt)
+hipError_t forward(tam_dn* n, const float* tgt)
 {
     hipError_t e;
     auto conv = [&](int l, const uint16_t* in, uint16_t* o) {
@@ -631,7 +632,7 @@ hipError_t forward(tam_dn* n, const uint16_t* tgt)
 }
 
 // backward after forward(tgt): gradients of the mean L1 into n->g
-hipError_t backward(tam_dn* n, const uint16_t* tgt)
+hipError_t backward(tam_dn* n, const float* tgt)
 {
     hipError_t e;
     hipLaunchKernelGGL(fill_f32k, dim3(grid_of(n->nparam)), dim3(256), 0, n->stream, n->g, n->nparam, 0.f);
@@ -793,7 +794,7 @@ int tam_dn_create(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t ca
     al((void**)&n->feat, 4ull * 5 * n->nv[0]);
     al((void**)&n->red, 16);
     al((void**)&n->ds_feat, 8ull * n->nv[0] * (capacity > 0 ? capacity : 1));
-    al((void**)&n->ds_tgt, 2ull * n->nv[0] * (capacity > 0 ? capacity : 1));
+    al((void**)&n->ds_tgt, 4ull * n->nv[0] * (capacity > 0 ? capacity : 1));
     if (e != hipSuccess) {
         g_dn_err = std::string("denoiser allocation: ") + hipGetErrorString(e);
         tam_dn_destroy(n);
@@ -927,7 +928,7 @@ int tam_dn_train(tam_dn* n, int32_t nslots, int32_t steps, double lr, uint64_t s
     hipError_t e = hipSuccess;
     for (int s = 0; s < steps && e == hipSuccess; ++s) {
         const int slot = (int)(rng() % (uint64_t)nslots);
-        const uint16_t* tgt = n->ds_tgt + (long)slot * nv;
+        const float* tgt = n->ds_tgt + (long)slot * nv;
         hipLaunchKernelGGL(pack_slot_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream,
                            n->ds_feat + (long)slot * nv * 4, nv, n->x0);
         e = forward(n, tgt);
@@ -953,7 +954,7 @@ int tam_dn_eval(tam_dn* n, int32_t slot, double* l1_pred, double* l1_input)
     if (slot < 0 || slot >= n->capacity) return dfail(n, TAM_ERR_ARG, "slot out of range");
     DevScope ds(n->dev);
     const long nv = n->nv[0];
-    const uint16_t* tgt = n->ds_tgt + (long)slot * nv;
+    const float* tgt = n->ds_tgt + (long)slot * nv;
     hipLaunchKernelGGL(pack_slot_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream, n->ds_feat + (long)slot * nv * 4,
                        nv, n->x0);
     DNCHK(n, hipEventRecord(n->ev0, n->stream));
@@ -965,6 +966,35 @@ int tam_dn_eval(tam_dn* n, int32_t slot, double* l1_pred, double* l1_input)
     n->last_ms = ms_between(n);
     if (l1_pred) *l1_pred = r[0] / (double)nv;
     if (l1_input) *l1_input = r[1] / (double)nv;
+    return TAM_OK;
+}
+
+int tam_dn_predict_slot(tam_dn* n, int32_t slot, float* pred, float* target, float* input)
+{
+    if (!n) return TAM_ERR_ARG;
+    if (slot < 0 || slot >= n->capacity) return dfail(n, TAM_ERR_ARG, "slot out of range");
+    DevScope ds(n->dev);
+    const long nv = n->nv[0];
+    hipLaunchKernelGGL(pack_slot_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream, n->ds_feat + (long)slot * nv * 4,
+                       nv, n->x0);
+    DNCHK(n, forward(n, nullptr));
+    if (pred) DNCHK(n, hipMemcpyAsync(pred, n->out, sizeof(float) * nv, hipMemcpyDeviceToHost, n->stream));
+    if (target)
+        DNCHK(n, hipMemcpyAsync(target, n->ds_tgt + (long)slot * nv, sizeof(float) * nv, hipMemcpyDeviceToHost,
+                                n->stream));
+    if (input) {
+        std::vector<uint16_t> f((size_t)nv * 4);
+        DNCHK(n, hipMemcpyAsync(f.data(), n->ds_feat + (long)slot * nv * 4, 8 * (size_t)nv, hipMemcpyDeviceToHost,
+                                n->stream));
+        DNCHK(n, hipStreamSynchronize(n->stream));
+        for (long m = 0; m < nv; ++m) {
+            const uint32_t u = (uint32_t)f[(size_t)m * 4] << 16;
+            float v;
+            std::memcpy(&v, &u, 4);
+            input[m] = v;
+        }
+    }
+    DNCHK(n, hipStreamSynchronize(n->stream));
     return TAM_OK;
 }
 
@@ -992,7 +1022,7 @@ int tam_dn_gradients_host(tam_dn* n, const float* features, const float* target,
     const long nv = n->nv[0];
     const int rc = tam_dn_store_host(n, 0, features, target);
     if (rc != TAM_OK) return rc;
-    const uint16_t* tgt = n->ds_tgt;
+    const float* tgt = n->ds_tgt;
     hipLaunchKernelGGL(pack_slot_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream, n->ds_feat, nv, n->x0);
     DNCHK(n, forward(n, tgt));
     DNCHK(n, backward(n, tgt));

@@ -12,7 +12,8 @@ from . import lib
 
 EXPORTED = ("tam_dn_create", "tam_dn_destroy", "tam_dn_last_error", "tam_dn_num_params", "tam_dn_get_params",
             "tam_dn_set_params", "tam_dn_store", "tam_dn_store_host", "tam_dn_train", "tam_dn_eval",
-            "tam_dn_forward_host", "tam_dn_gradients_host", "tam_dn_apply", "tam_dn_last_ms")
+            "tam_dn_forward_host", "tam_dn_gradients_host", "tam_dn_apply", "tam_dn_last_ms",
+            "tam_dn_predict_slot")
 
 # layer table of the fixed U-Net: (cin, cout, level)
 LAYERS = ((16, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2), (96, 32, 1),
@@ -42,6 +43,7 @@ def _bind():
     L.tam_dn_train.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.c_uint64, vp]
     L.tam_dn_eval.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.tam_dn_forward_host.argtypes = [vp, vp, vp]
+    L.tam_dn_predict_slot.argtypes = [vp, C.c_int32, vp, vp, vp]
     L.tam_dn_gradients_host.argtypes = [vp, vp, vp, vp, C.POINTER(C.c_double)]
     L.tam_dn_apply.argtypes = [vp, vp, C.c_double]
     L.tam_dn_last_ms.argtypes = [vp]
@@ -122,6 +124,12 @@ class Denoiser:
         a, b = C.c_double(), C.c_double()
         self._chk(self.L.tam_dn_eval(self.h, slot, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def predict_slot(self, slot):
+        """(prediction, target, noisy input), each log(1 + phi) (nz, ny, nx)."""
+        a, b, c = (np.empty(self.shape, np.float32) for _ in range(3))
+        self._chk(self.L.tam_dn_predict_slot(self.h, slot, a.ctypes.data, b.ctypes.data, c.ctypes.data))
+        return a, b, c
 
     def forward(self, features):
         f = _f32(features)
