@@ -226,6 +226,10 @@ int tam_cycle(tam_ctx* ctx, int64_t npackets, uint64_t seed, uint32_t stream, do
               double arr_delt, tam_pulse* pulse, tam_mc_stats* stats);
 
 /* --- data movement ---------------------------------------------------- */
+/* Sums of the transport tally over k^3 blocks of voxels (k divides every
+ * edge), x-block fastest: a cheap device-side summary for checks that
+ * compare two transport runs (no reference counterpart). */
+int tam_tally_blocks(tam_ctx* ctx, int32_t k, double* sums);
 int tam_download(tam_ctx* ctx, int32_t field, double* out);
 int tam_upload(tam_ctx* ctx, int32_t field, const double* in);
 

@@ -914,6 +914,22 @@ int tam_cycle(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream, doub
     return rc;
 }
 
+int tam_tally_blocks(tam_ctx* c, int32_t k, double* sums)
+{
+    if (!c || !sums) return TAM_ERR_ARG;
+    if (k < 1 || c->p.nx % k || c->p.ny % k || c->p.nz % k) return fail(c, TAM_ERR_ARG, "k must divide every grid edge");
+    Dev d(c->dev);
+    const long nb = (long)(c->p.nx / k) * (c->p.ny / k) * (c->p.nz / k);
+    double* tmp = nullptr;
+    HIPCHK(c, hipMalloc((void**)&tmp, sizeof(double) * (size_t)nb));
+    hipError_t e = launch_block_sums(c->tally, c->p.nx, c->p.ny, c->p.nz, k, tmp, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(sums, tmp, sizeof(double) * (size_t)nb, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return hip_fail(c, "tally blocks", e);
+    return TAM_OK;
+}
+
 int tam_download(tam_ctx* c, int32_t field, double* out)
 {
     if (!c) return TAM_ERR_ARG;

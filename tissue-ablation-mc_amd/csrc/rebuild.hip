@@ -253,6 +253,33 @@ hipError_t launch_normalise(int nx, int ny, int nz, long px, long pxy, const dou
                        pxy, tally, rhokap, mat, mua_frac, dyn, scale, coeff, jmean, src);
     return hipGetLastError();
 }
+// k^3 block sums of an interior FP64 field: one workgroup per block
+__global__ __launch_bounds__(256) void block_sum_kernel(const double* f, int nx, int ny, int k, double* out)
+{
+    __shared__ double red[256];
+    const int bx = nx / k, by = ny / k;
+    const long b = blockIdx.x;
+    const int ib = (int)(b % bx), jb = (int)((b / bx) % by), kb = (int)(b / ((long)bx * by));
+    double s = 0.0;
+    for (int q = threadIdx.x; q < k * k * k; q += 256) {
+        const int i = q % k, j = (q / k) % k, l = q / (k * k);
+        s += f[(long)(ib * k + i) + (long)nx * ((jb * k + j) + (long)ny * (kb * k + l))];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[b] = red[0];
+}
+
+hipError_t launch_block_sums(const double* f, int nx, int ny, int nz, int k, double* out, hipStream_t s)
+{
+    const long nb = (long)(nx / k) * (ny / k) * (nz / k);
+    hipLaunchKernelGGL(block_sum_kernel, dim3((unsigned)nb), dim3(256), 0, s, f, nx, ny, k, out);
+    return hipGetLastError();
+}
 hipError_t launch_energy(int nx, int ny, int nz, long px, long pxy, const float* dT, const double* T64,
                          double t_ref, const double* heatcap, double mass_voxel, double* out, hipStream_t s)
 {

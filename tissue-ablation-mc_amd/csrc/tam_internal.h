@@ -250,5 +250,7 @@ hipError_t launch_fill_f64(double* p, long n, double v, hipStream_t s);
 hipError_t launch_fill_f32(float* p, long n, float v, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, long n, hipStream_t s);
 hipError_t launch_copy(const void* in, void* out, long bytes, hipStream_t s);
+// k^3 block sums of an interior field into out[(nz/k)(ny/k)(nx/k)] (k divides every edge)
+hipError_t launch_block_sums(const double* f, int nx, int ny, int nz, int k, double* out, hipStream_t s);
 hipError_t launch_energy(int nx, int ny, int nz, long px, long pxy, const float* dT, const double* T64,
                          double t_ref, const double* heatcap, double mass_voxel, double* out, hipStream_t s);

@@ -27,7 +27,8 @@ EXPORTED = ("tam_abi_version", "tam_create", "tam_destroy", "tam_last_error", "t
             "tam_set_thermal", "tam_mc_run", "tam_normalise", "tam_heat_run", "tam_arrhenius",
             "tam_rebuild", "tam_download", "tam_upload", "tam_energy", "tam_get_timing",
             "tam_synchronize", "tam_set_heat_launch", "tam_tune_heat", "tam_set_mc_launch", "tam_tune_mc",
-            "tam_copy_bandwidth", "tam_atomic_rate", "tam_struct_size", "tam_pulse_advance", "tam_cycle")
+            "tam_copy_bandwidth", "tam_atomic_rate", "tam_struct_size", "tam_pulse_advance", "tam_cycle",
+            "tam_tally_blocks")
 
 
 class Params(C.Structure):
@@ -125,6 +126,7 @@ def load():
     L.tam_set_mc_launch.argtypes = [vp, C.c_int32, C.c_int32]
     L.tam_copy_bandwidth.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
     L.tam_atomic_rate.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(C.c_double)]
+    L.tam_tally_blocks.argtypes = [vp, C.c_int32, dp]
     L.tam_struct_size.argtypes = [C.c_int32]
     L.tam_struct_size.restype = C.c_int64
     L.tam_pulse_advance.argtypes = [C.POINTER(Pulse), C.c_double]
@@ -249,6 +251,13 @@ class TissueGrid:
         t = Timing()
         self._chk(self.L.tam_get_timing(self.h, C.byref(t)))
         return t
+
+    def tally_blocks(self, k):
+        """k^3 block sums of the transport tally, shape (nz/k, ny/k, nx/k)."""
+        nz, ny, nx = self.shape
+        out = np.empty((nz // k, ny // k, nx // k), dtype=np.float64)
+        self._chk(self.L.tam_tally_blocks(self.h, int(k), _ptr(out)))
+        return out
 
     def synchronize(self):
         self._chk(self.L.tam_synchronize(self.h))
