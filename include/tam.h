@@ -252,6 +252,33 @@ int tam_atomic_rate(tam_ctx* ctx, int64_t count, int64_t slots, double* per_s);
  * 3dFD.f90:179-182), which asynchronous tam_cycle calls cannot report. */
 int tam_synchronize(tam_ctx* ctx);
 
+/* --- batches of independent instances (no reference counterpart: the
+ * reference runs one tissue per job).  A batch refers to existing contexts
+ * on one device (it does not own them); every instance keeps its own
+ * arrays.  Instances run the FP32 step without boiling or in-step damage,
+ * with one convective-top setting and one coefficient mode.
+ * tam_batch_mc_run: the transport of all instances in one launch (instance
+ *   i: npackets[i] packets on Philox stream stream0 + i of key seed, as
+ *   tam_mc_run would give); stats[i] per instance when stats is not NULL.
+ * tam_batch_heat_run: `loops` steps of every instance, one launch per step
+ *   (laser on; the source from each instance's last tam_normalise).
+ * tam_batch_tune: times transport shapes (blocks per CU x random-phase
+ *   batch) on sample_packets[i] per instance and stencil shapes (direct
+ *   tiles 3-5 x z depths) over heat_reps steps without advancing the state,
+ *   and keeps the fastest. */
+typedef struct tam_batch tam_batch;
+int  tam_batch_create(tam_ctx* const* ctxs, int32_t n, tam_batch** out);
+void tam_batch_destroy(tam_batch* b);
+const char* tam_batch_last_error(const tam_batch* b);
+int  tam_batch_mc_run(tam_batch* b, const int64_t* npackets, uint64_t seed, uint32_t stream0, int32_t accumulate,
+                      tam_mc_stats* stats);
+int  tam_batch_heat_run(tam_batch* b, int32_t loops);
+int  tam_batch_set_launch(tam_batch* b, int32_t tile, int32_t kz, int32_t blocks_per_cu, int32_t rng_batch);
+int  tam_batch_tune(tam_batch* b, const int64_t* sample_packets, int32_t heat_reps, int32_t* tile, int32_t* kz,
+                    int32_t* blocks_per_cu, int32_t* rng_batch);
+/* device time of the last batched launch sequence, ms */
+double tam_batch_last_ms(const tam_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

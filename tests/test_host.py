@@ -62,3 +62,16 @@ def test_config0_thermal_and_time_step():
     assert math.isclose(p.delt, 9.74e-3, rel_tol=2e-3)
     assert p.robin_top == 1 and p.h_top == 10.0 and p.n_ext_zhi == 1.0
     assert np.all(prob.temp == 310.15)
+
+
+def test_batch_manifest_is_the_committed_one():
+    """The 64-instance manifest is drawn from a fixed PCG64 seed; the JSON
+    beside the module is that draw (edges, optics and packet counts in the
+    configured ranges)."""
+    import json
+    m = configs.batch_manifest()
+    path = os.path.join(os.path.dirname(configs.__file__), "batch_manifest.json")
+    assert json.load(open(path)) == m
+    assert len(m) == 64 and {x["edge"] for x in m} <= {64, 96, 128, 160}
+    assert all(0.1 <= x["mua"] <= 10 and 50 <= x["mus"] <= 300 and 0.7 <= x["g"] <= 0.95 for x in m)
+    assert all(1e6 <= x["packets"] <= 1e7 for x in m)

@@ -1237,3 +1237,284 @@ int tam_synchronize(tam_ctx* c)
 }
 
 }  // extern "C"
+
+// ---- batches of instances ------------------------------------------------
+
+struct tam_batch {
+    std::vector<tam_ctx*> ctx;
+    int dev;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;
+    std::string err;
+    McArgs* dmc;
+    HeatBatchEntry* dheat;
+    int* dtile;
+    int tile, kz, bpc, rng_batch;
+    double last_ms;
+};
+
+namespace {
+
+int bfail(tam_batch* b, int code, const std::string& m)
+{
+    b->err = m;
+    return code;
+}
+#define BCHK(b, call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) return bfail((b), TAM_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+thread_local std::string g_batch_err;
+
+float bms(tam_batch* b)
+{
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    return ms;
+}
+
+int sync_ctxs(tam_batch* b)
+{
+    for (tam_ctx* c : b->ctx) BCHK(b, hipStreamSynchronize(c->stream));
+    return TAM_OK;
+}
+
+// one batched transport launch (counters of every instance cleared first)
+int batch_mc(tam_batch* b, const int64_t* npk, uint64_t seed, uint32_t stream0, int accumulate)
+{
+    const int n = (int)b->ctx.size();
+    std::vector<McArgs> tab((size_t)n);
+    long total = 0;
+    for (int i = 0; i < n; ++i) {
+        tam_ctx* c = b->ctx[(size_t)i];
+        if (!accumulate) BCHK(b, launch_fill_f64(c->tally, c->nvox, 0.0, b->stream));
+        BCHK(b, launch_fill_u64(c->counters, TAM_CNT_SLOTS, b->stream));
+        tab[(size_t)i] = mc_args(c, npk[i], seed, stream0 + (uint32_t)i);
+        tab[(size_t)i].blocks_per_cu = b->bpc;
+        tab[(size_t)i].rng_batch = b->rng_batch;
+        total += (long)npk[i];
+        c->src_valid = false;
+    }
+    BCHK(b, hipMemcpyAsync(b->dmc, tab.data(), sizeof(McArgs) * (size_t)n, hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipEventRecord(b->ev0, b->stream));
+    BCHK(b, launch_mc_batch(b->dmc, n, total, b->bpc, b->stream));
+    BCHK(b, hipEventRecord(b->ev1, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    b->last_ms = bms(b);
+    return TAM_OK;
+}
+
+// tile table for the batch's shape; returns the total tile count
+int batch_tiles(tam_batch* b, int tile, int kz, std::vector<int>& start)
+{
+    int tx = 0, ty = 0;
+    heat_tile_dims(tile, &tx, &ty);
+    const int n = (int)b->ctx.size();
+    start.assign((size_t)n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        const tam_params& p = b->ctx[(size_t)i]->p;
+        const long t = (long)((p.nx + tx - 1) / tx) * ((p.ny + ty - 1) / ty) * ((p.nz + kz - 1) / kz);
+        start[(size_t)i + 1] = start[(size_t)i] + (int)t;
+    }
+    return start[(size_t)n];
+}
+
+// `steps` batched heat steps with the given shape; advance = 0 keeps the
+// state (every step reads cur and writes nxt, no energy accounting)
+int batch_heat(tam_batch* b, int steps, int tile, int kz, bool advance)
+{
+    const int n = (int)b->ctx.size();
+    std::vector<HeatBatchEntry> tab((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        tam_ctx* c = b->ctx[(size_t)i];
+        tab[(size_t)i].a = heat_args(c);
+        tab[(size_t)i].cur = c->cur;
+        tab[(size_t)i].nxt = c->nxt;
+    }
+    std::vector<int> start;
+    const int total = batch_tiles(b, tile, kz, start);
+    BCHK(b, hipMemcpyAsync(b->dheat, tab.data(), sizeof(HeatBatchEntry) * (size_t)n, hipMemcpyHostToDevice,
+                           b->stream));
+    BCHK(b, hipMemcpyAsync(b->dtile, start.data(), sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice, b->stream));
+    const tam_params& p0 = b->ctx[0]->p;
+    const int robin = p0.robin_top != 0, lut = b->ctx[0]->lut ? 1 : 0;
+    BCHK(b, hipEventRecord(b->ev0, b->stream));
+    for (int k = 0; k < steps; ++k)
+        BCHK(b, launch_heat_batch(b->dheat, n, b->dtile, total, 1.f, 0.0, 0, robin, 0, advance ? 1 : 0, lut, tile,
+                                  kz, advance ? (k & 1) : 0, b->stream));
+    BCHK(b, hipEventRecord(b->ev1, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    b->last_ms = bms(b);
+    if (advance) {
+        for (tam_ctx* c : b->ctx) {
+            if (steps & 1) std::swap(c->cur, c->nxt);
+            c->time += steps * c->p.delt;
+            c->timing.heat_steps = steps;
+        }
+    }
+    return TAM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tam_batch_create(tam_ctx* const* ctxs, int32_t n, tam_batch** out)
+{
+    if (!ctxs || !out || n < 1) return TAM_ERR_ARG;
+    *out = nullptr;
+    const tam_ctx* c0 = ctxs[0];
+    for (int i = 0; i < n; ++i) {
+        const tam_ctx* c = ctxs[i];
+        if (!c || c->dev != c0->dev) { g_batch_err = "instances must share one device"; return TAM_ERR_ARG; }
+        if (!c->optics_set || !c->thermal_set) { g_batch_err = "instances need optics and thermal state"; return TAM_ERR_STATE; }
+        if (c->fp64 || c->p.boil || c->p.damage_every_step || c->p.robin_top != c0->p.robin_top || c->lut != c0->lut) {
+            g_batch_err = "batched instances run the FP32 step without boiling or in-step damage, "
+                          "with one convective-top setting and one coefficient mode";
+            return TAM_ERR_ARG;
+        }
+    }
+    tam_batch* b = new (std::nothrow) tam_batch();
+    if (!b) return TAM_ERR_NOMEM;
+    b->ctx.assign(ctxs, ctxs + n);
+    b->dev = c0->dev;
+    b->tile = 4;
+    b->kz = 8;
+    b->bpc = 4;
+    b->rng_batch = 32;
+    Dev d(b->dev);
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->dmc, sizeof(McArgs) * (size_t)n);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->dheat, sizeof(HeatBatchEntry) * (size_t)n);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->dtile, sizeof(int) * ((size_t)n + 1));
+    if (e != hipSuccess) {
+        g_batch_err = std::string("batch allocation: ") + hipGetErrorString(e);
+        tam_batch_destroy(b);
+        return TAM_ERR_HIP;
+    }
+    *out = b;
+    return TAM_OK;
+}
+
+void tam_batch_destroy(tam_batch* b)
+{
+    if (!b) return;
+    Dev d(b->dev);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->dmc) hipFree(b->dmc);
+    if (b->dheat) hipFree(b->dheat);
+    if (b->dtile) hipFree(b->dtile);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char* tam_batch_last_error(const tam_batch* b) { return b ? b->err.c_str() : g_batch_err.c_str(); }
+
+int tam_batch_mc_run(tam_batch* b, const int64_t* npackets, uint64_t seed, uint32_t stream0, int32_t accumulate,
+                     tam_mc_stats* stats)
+{
+    if (!b || !npackets) return TAM_ERR_ARG;
+    const int n = (int)b->ctx.size();
+    for (int i = 0; i < n; ++i)
+        if (npackets[i] < 0) return bfail(b, TAM_ERR_ARG, "negative packet count");
+    Dev d(b->dev);
+    int rc = sync_ctxs(b);
+    if (rc == TAM_OK) rc = batch_mc(b, npackets, seed, stream0, accumulate);
+    if (rc != TAM_OK || !stats) return rc;
+    for (int i = 0; i < n; ++i) {
+        unsigned long long cnt[TAM_CNT_N];
+        BCHK(b, hipMemcpy(cnt, b->ctx[(size_t)i]->counters, sizeof(cnt), hipMemcpyDeviceToHost));
+        tam_mc_stats& s = stats[i];
+        std::memset(&s, 0, sizeof(s));
+        s.packets = (double)npackets[i];
+        s.absorbed = (double)cnt[TAM_CNT_ABSORBED];
+        s.specular = (double)cnt[TAM_CNT_SPECULAR];
+        for (int f = 0; f < 6; ++f) s.escaped[f] = (double)cnt[TAM_CNT_ESC0 + f];
+        s.missed = (double)cnt[TAM_CNT_MISSED];
+        s.crossings = (double)cnt[TAM_CNT_CROSS];
+        s.interactions = (double)cnt[TAM_CNT_INTER];
+        s.killed = (double)cnt[TAM_CNT_KILLED];
+        s.tally_adds = (double)cnt[TAM_CNT_TALLY];
+        s.kernel_ms = b->last_ms;
+    }
+    return TAM_OK;
+}
+
+int tam_batch_heat_run(tam_batch* b, int32_t loops)
+{
+    if (!b || loops < 0) return TAM_ERR_ARG;
+    Dev d(b->dev);
+    int rc = sync_ctxs(b);
+    if (rc == TAM_OK) rc = batch_heat(b, loops, b->tile, b->kz, true);
+    if (rc != TAM_OK) return rc;
+    for (tam_ctx* c : b->ctx) {
+        c->timing.heat_ms = b->last_ms;
+        const int r = check_flags(c);
+        if (r != TAM_OK) return bfail(b, r, c->err);
+    }
+    return TAM_OK;
+}
+
+int tam_batch_set_launch(tam_batch* b, int32_t tile, int32_t kz, int32_t blocks_per_cu, int32_t rng_batch)
+{
+    if (!b) return TAM_ERR_ARG;
+    if (tile < 3 || tile > 5 || kz < 1 || blocks_per_cu < 1 || blocks_per_cu > 64 || rng_batch < 1 || rng_batch > 64)
+        return bfail(b, TAM_ERR_ARG, "batch launch: tile 3..5, kz >= 1, blocks_per_cu 1..64, rng_batch 1..64");
+    b->tile = tile;
+    b->kz = kz;
+    b->bpc = blocks_per_cu;
+    b->rng_batch = rng_batch;
+    return TAM_OK;
+}
+
+int tam_batch_tune(tam_batch* b, const int64_t* sample_packets, int32_t heat_reps, int32_t* tile, int32_t* kz,
+                   int32_t* blocks_per_cu, int32_t* rng_batch)
+{
+    if (!b || !sample_packets) return TAM_ERR_ARG;
+    if (heat_reps < 1) heat_reps = 1;
+    Dev d(b->dev);
+    int rc = sync_ctxs(b);
+    if (rc != TAM_OK) return rc;
+    // transport: blocks per CU x random-phase batch on the sample counts
+    double best = 1e300;
+    int bb = b->bpc, bt = b->rng_batch;
+    for (int bpc : {2, 4, 8})
+        for (int rb : {16, 32, 48}) {
+            b->bpc = bpc;
+            b->rng_batch = rb;
+            rc = batch_mc(b, sample_packets, 0x5eed5eedULL, 0xfffff000u, 0);
+            if (rc != TAM_OK) return rc;
+            if (b->last_ms < best) { best = b->last_ms; bb = bpc; bt = rb; }
+        }
+    b->bpc = bb;
+    b->rng_batch = bt;
+    for (tam_ctx* c : b->ctx) BCHK(b, launch_fill_f64(c->tally, c->nvox, 0.0, b->stream));
+    // stencil: direct shapes x z depths, state not advanced
+    best = 1e300;
+    int btl = b->tile, bk = b->kz;
+    for (int t = 3; t <= 5; ++t)
+        for (int k : {4, 8, 16, 32}) {
+            rc = batch_heat(b, 1, t, k, false);            // warm-up
+            if (rc == TAM_OK) rc = batch_heat(b, heat_reps, t, k, false);
+            if (rc != TAM_OK) return rc;
+            if (b->last_ms < best) { best = b->last_ms; btl = t; bk = k; }
+        }
+    b->tile = btl;
+    b->kz = bk;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    if (tile) *tile = btl;
+    if (kz) *kz = bk;
+    if (blocks_per_cu) *blocks_per_cu = bb;
+    if (rng_batch) *rng_batch = bt;
+    return TAM_OK;
+}
+
+double tam_batch_last_ms(const tam_batch* b) { return b ? b->last_ms : 0.0; }
+
+}  // extern "C"

@@ -288,15 +288,12 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
 // its own +-x/+-y neighbours (they share cache lines with the tile's own
 // rows, so they are L1/L2 hits) from addresses clamped into the interior, so
 // the plane loop is straight-line code; +-z stay in registers.
+// tl: this block's tile; gx, gy: tiles per row and column of the grid.
 template <int TX, int TY, bool LUT>
-__global__ __launch_bounds__(TX* TY) void stencil_direct_kernel(const HeatArgs a, const float* __restrict__ Tin,
-                                                                float* __restrict__ Tout, float lon,
-                                                                double t_after, int kz, unsigned flags)
+__device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __restrict__ Tin,
+                                            float* __restrict__ Tout, float lon, double t_after, int kz,
+                                            unsigned flags, const TileId tl, int gx, int gy)
 {
-    if (a.dyn_lon) {              // graph replay: this step's values from device memory
-        lon = *a.dyn_lon;
-        t_after = *a.dyn_tafter;
-    }
     __shared__ float sLut[LUT ? 3 * 256 + 16 : 1];   // face table + heatcap*mass per class
     __shared__ float2 sRob[LUT ? 16 : 1];
     const float* __restrict__ gxa = a.gx;
@@ -307,7 +304,6 @@ __global__ __launch_bounds__(TX* TY) void stencil_direct_kernel(const HeatArgs a
     const long px = a.px, pxy = a.pxy;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const TileId tl = xcd_tile();
     const int i = tl.x * TX + tx + 1, j = tl.y * TY + ty + 1;
     const bool in = i <= nx && j <= ny;
     const bool bnd_ij = i == 1 || i == nx || j == 1 || j == ny;     // lane on an x or y face
@@ -316,8 +312,7 @@ __global__ __launch_bounds__(TX* TY) void stencil_direct_kernel(const HeatArgs a
     const int k1 = min(k0 + kz, nz + 1);
     const bool robin = flags & F_ROBIN, damage = flags & F_DAMAGE, boil = flags & F_BOIL;
     const bool energy = (flags & F_ENERGY) &&
-                        (tl.x == 0 || tl.x == (int)gridDim.x - 1 || tl.y == 0 ||
-                         tl.y == (int)gridDim.y - 1 || k0 == 1 || k1 == nz + 1);
+                        (tl.x == 0 || tl.x == gx - 1 || tl.y == 0 || tl.y == gy - 1 || k0 == 1 || k1 == nz + 1);
     if (LUT) {
         for (int t = threadIdx.x; t < 3 * 256 + 16; t += TX * TY) sLut[t] = a.glut[t];
         if (threadIdx.x < 16) sRob[threadIdx.x] = a.robin_lut[threadIdx.x];
@@ -429,6 +424,46 @@ __global__ __launch_bounds__(TX* TY) void stencil_direct_kernel(const HeatArgs a
     }
 }
 
+template <int TX, int TY, bool LUT>
+__global__ __launch_bounds__(TX* TY) void stencil_direct_kernel(const HeatArgs a, const float* __restrict__ Tin,
+                                                                float* __restrict__ Tout, float lon,
+                                                                double t_after, int kz, unsigned flags)
+{
+    if (a.dyn_lon) {              // graph replay: this step's values from device memory
+        lon = *a.dyn_lon;
+        t_after = *a.dyn_tafter;
+    }
+    direct_body<TX, TY, LUT>(a, Tin, Tout, lon, t_after, kz, flags, xcd_tile(), (int)gridDim.x, (int)gridDim.y);
+}
+
+// Many grids in one launch: block b works on the instance whose tile range
+// [tile_start[i], tile_start[i + 1]) holds b, at that instance's local tile
+// (x fastest).  Every instance uses the same tile shape and step flags.
+template <int TX, int TY, bool LUT>
+__global__ __launch_bounds__(TX* TY) void stencil_batch_kernel(const HeatBatchEntry* __restrict__ tab, int n,
+                                                               const int* __restrict__ tile_start, float lon,
+                                                               double t_after, int kz, unsigned flags, int parity)
+{
+    const int b = blockIdx.x;
+    int lo = 0, hi = n - 1;                       // last i with tile_start[i] <= b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const HeatBatchEntry& e = tab[lo];
+    const int gx = (e.a.nx + TX - 1) / TX, gy = (e.a.ny + TY - 1) / TY;
+    const int local = b - tile_start[lo];
+    TileId tl;
+    tl.x = local % gx;
+    tl.y = (local / gx) % gy;
+    tl.z = local / (gx * gy);
+    tl.id = local;
+    const float* Tin = parity ? e.nxt : e.cur;
+    float* Tout = parity ? e.cur : e.nxt;
+    direct_body<TX, TY, LUT>(e.a, Tin, Tout, lon, t_after, kz, flags, tl, gx, gy);
+}
+
 // Arrhenius at the reference cadence: every interior voxel, the full
 // threshold chain (3dFD.f90:447-464).
 __global__ __launch_bounds__(256) void arrhenius_kernel(HeatArgs a, const float* __restrict__ T,
@@ -472,7 +507,42 @@ hipError_t launch_t(const HeatArgs& a, const float* Tin, float* Tout, float lon,
     return hipGetLastError();
 }
 
+template <int TX, int TY, bool LUT>
+hipError_t launch_batch_t(const HeatBatchEntry* tab, int n, const int* tile_start, int total_tiles, float lon,
+                          double t, int kz, unsigned flags, int parity, hipStream_t s)
+{
+    hipLaunchKernelGGL((stencil_batch_kernel<TX, TY, LUT>), dim3((unsigned)total_tiles), dim3(TX * TY), 0, s, tab, n,
+                       tile_start, lon, t, kz, flags, parity);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+int heat_tile_dims(int tile, int* tx, int* ty)
+{
+    static const int d[6][2] = {{64, 4}, {32, 8}, {64, 8}, {64, 4}, {32, 8}, {64, 8}};
+    if (tile < 0 || tile > 5) return -1;
+    *tx = d[tile][0];
+    *ty = d[tile][1];
+    return 0;
+}
+
+hipError_t launch_heat_batch(const HeatBatchEntry* tab, int n, const int* tile_start, int total_tiles, float lon,
+                             double time_after, int boil, int robin, int damage, int energy, int lut, int tile,
+                             int kz, int parity, hipStream_t s)
+{
+    const unsigned f = (boil ? F_BOIL : 0u) | (robin ? F_ROBIN : 0u) | (damage ? F_DAMAGE : 0u) |
+                       (energy ? F_ENERGY : 0u);
+    switch (tile * 2 + (lut ? 1 : 0)) {
+    case 6: return launch_batch_t<64, 4, false>(tab, n, tile_start, total_tiles, lon, time_after, kz, f, parity, s);
+    case 7: return launch_batch_t<64, 4, true>(tab, n, tile_start, total_tiles, lon, time_after, kz, f, parity, s);
+    case 8: return launch_batch_t<32, 8, false>(tab, n, tile_start, total_tiles, lon, time_after, kz, f, parity, s);
+    case 9: return launch_batch_t<32, 8, true>(tab, n, tile_start, total_tiles, lon, time_after, kz, f, parity, s);
+    case 10: return launch_batch_t<64, 8, false>(tab, n, tile_start, total_tiles, lon, time_after, kz, f, parity, s);
+    case 11: return launch_batch_t<64, 8, true>(tab, n, tile_start, total_tiles, lon, time_after, kz, f, parity, s);
+    }
+    return hipErrorInvalidValue;      // batches use the direct shapes 3-5
+}
 
 // launch shapes: 0-2 stage x-y planes in LDS, 3-5 load neighbours directly
 int heat_num_tiles() { return 6; }

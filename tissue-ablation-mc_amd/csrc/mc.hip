@@ -111,7 +111,9 @@ constexpr uint32_t kChunk = 256;            // packets a wave takes from the que
 
 enum : int { PEND_NONE = 0, PEND_LAUNCH = 1, PEND_INTERACT = 2, PEND_FRESNEL = 3 };
 
-__global__ __launch_bounds__(256) void mc_kernel(McArgs a)
+// The transport of one instance by the calling wave's lanes, until its
+// packet queue is drained; the fate counters go to a.counters.
+__device__ __forceinline__ void transport(const McArgs& a)
 {
     unsigned long long* head = a.counters + TAM_CNT_N;     // queue head, packets taken
     const uint32_t nxy = (uint32_t)a.nx * (uint32_t)a.ny;
@@ -370,6 +372,24 @@ __global__ __launch_bounds__(256) void mc_kernel(McArgs a)
     }
 }
 
+__global__ __launch_bounds__(256) void mc_kernel(McArgs a) { transport(a); }
+
+// Many instances in one launch: a wave takes packets from one instance's
+// queue until it drains, then moves on to the next instance (starting at its
+// own index, so the waves spread over the instances); each instance keeps its
+// own queue, tally and counters, and the instance's arguments stay uniform
+// over the wave.
+__global__ __launch_bounds__(256) void mc_batch_kernel(const McArgs* __restrict__ tab, int n)
+{
+    const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    for (int k = 0; k < n; ++k) {
+        const int i = (wave + k) % n;
+        const McArgs a = tab[i];
+        if (a.npk == 0) continue;
+        transport(a);
+    }
+}
+
 // Atomic-rate probe: each lane adds 1.0 to `per_lane` xorshift-chosen slots.
 __global__ __launch_bounds__(256) void atomic_rate_kernel(double* t, uint32_t slots, uint32_t per_lane)
 {
@@ -392,6 +412,20 @@ hipError_t launch_atomic_rate(double* t, long slots, long count, hipStream_t s)
     if (per < 1) per = 1;
     if (slots < 1 || slots > 0xffffffffL) return hipErrorInvalidValue;
     hipLaunchKernelGGL(atomic_rate_kernel, dim3(2048), dim3(256), 0, s, t, (uint32_t)slots, (uint32_t)per);
+    return hipGetLastError();
+}
+
+hipError_t launch_mc_batch(const McArgs* dtab, int n, long total_npk, int blocks_per_cu, hipStream_t s)
+{
+    if (n <= 0 || total_npk <= 0) return hipSuccess;
+    int dev = 0, ncu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess || ncu <= 0) ncu = 256;
+    long blocks = (long)ncu * (blocks_per_cu > 0 ? blocks_per_cu : 8);
+    const long need = (total_npk + 255) / 256;
+    if (need < blocks) blocks = need;
+    hipLaunchKernelGGL(mc_batch_kernel, dim3((unsigned)blocks), dim3(256), 0, s, dtab, n);
     return hipGetLastError();
 }
 

@@ -96,6 +96,13 @@ struct HeatArgs {
     const double* dyn_arr;
 };
 
+// one instance of a batched heat step: its arguments and its ping-pong pair
+struct HeatBatchEntry {
+    HeatArgs a;
+    float* cur;                // input at even parity (output at odd)
+    float* nxt;
+};
+
 #define TAM_NCLASS 16          // LUT classes; the last one is "removed" (no conduction)
 #define TAM_CLASS_REMOVED (TAM_NCLASS - 1)
 
@@ -223,12 +230,20 @@ static_assert(sizeof(CycleHdr) == 64, "cycle header layout");
 
 // ---- launchers (each enqueues on `s`, returns hipError_t) --------------
 hipError_t launch_mc(const McArgs& a, hipStream_t s);
+// all instances of a device table in one launch (see mc_batch_kernel)
+hipError_t launch_mc_batch(const McArgs* dtab, int n, long total_npk, int blocks_per_cu, hipStream_t s);
 // `count` FP64 atomic adds (whole waves) to pseudo-random slots of t[0, slots)
 hipError_t launch_atomic_rate(double* t, long slots, long count, hipStream_t s);
 hipError_t launch_heat_step(const HeatArgs& a, const float* Tin, float* Tout, float lon, double time_after,
                             int boil, int robin, int damage, int energy, int lut, int tile, int kz,
                             hipStream_t s);
 int heat_num_tiles();
+int heat_tile_dims(int tile, int* tx, int* ty);
+// one step of every instance of a table (direct shapes 3-5 only); parity 0
+// reads cur and writes nxt, parity 1 the reverse
+hipError_t launch_heat_batch(const HeatBatchEntry* tab, int n, const int* tile_start, int total_tiles, float lon,
+                             double time_after, int boil, int robin, int damage, int energy, int lut, int tile,
+                             int kz, int parity, hipStream_t s);
 const char* heat_tile_name(int tile);
 hipError_t launch_heat64_step(const Heat64Args& a, const double* Tin, double* Tout, double lon,
                               double time_after, int boil, int robin, int damage, int energy, hipStream_t s);

@@ -270,3 +270,27 @@ def homogeneous(n, mua, mus, g, nidx=1.38, ext=1.0, packets=1_000_000, steps=200
     temp, kappa, density, heatcap = _thermal_cube(p, C0_THERMAL)
     return Problem(f"homogeneous-{n}", p, mat, table, temp, kappa, density, heatcap,
                    packets=packets, steps=steps, power=1.0)
+
+
+def batch_manifest(n=64, seed=20240611):
+    """The batched-instances configuration: n homogeneous tissues with grid
+    edge from {64, 96, 128, 160}, mu_a log-uniform in [0.1, 10] /cm, mu_s
+    log-uniform in [50, 300] /cm, g uniform in [0.7, 0.95], n = 1.38, packets
+    log-uniform in [1e6, 1e7]; drawn once from numpy's PCG64 with a fixed seed."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n):
+        out.append(dict(edge=int(rng.choice([64, 96, 128, 160])),
+                        mua=float(np.exp(rng.uniform(np.log(0.1), np.log(10.0)))),
+                        mus=float(np.exp(rng.uniform(np.log(50.0), np.log(300.0)))),
+                        g=float(rng.uniform(0.7, 0.95)),
+                        packets=int(round(np.exp(rng.uniform(np.log(1e6), np.log(1e7)))))))
+    return out
+
+
+def batch_problems(manifest, steps=2000):
+    """One Problem per manifest entry: a 1 cm cube with config-0 thermal
+    constants (convective top, 37 C elsewhere), top-hat beam of radius
+    0.1 cm, 1 W."""
+    return [homogeneous(m["edge"], m["mua"], m["mus"], m["g"], packets=m["packets"], steps=steps)
+            for m in manifest]

@@ -28,7 +28,8 @@ EXPORTED = ("tam_abi_version", "tam_create", "tam_destroy", "tam_last_error", "t
             "tam_rebuild", "tam_download", "tam_upload", "tam_energy", "tam_get_timing",
             "tam_synchronize", "tam_set_heat_launch", "tam_tune_heat", "tam_set_mc_launch", "tam_tune_mc",
             "tam_copy_bandwidth", "tam_atomic_rate", "tam_struct_size", "tam_pulse_advance", "tam_cycle",
-            "tam_tally_blocks")
+            "tam_tally_blocks", "tam_batch_create", "tam_batch_destroy", "tam_batch_last_error",
+            "tam_batch_mc_run", "tam_batch_heat_run", "tam_batch_set_launch", "tam_batch_tune", "tam_batch_last_ms")
 
 
 class Params(C.Structure):
@@ -127,6 +128,18 @@ def load():
     L.tam_copy_bandwidth.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
     L.tam_atomic_rate.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(C.c_double)]
     L.tam_tally_blocks.argtypes = [vp, C.c_int32, dp]
+    L.tam_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
+    L.tam_batch_destroy.argtypes = [vp]
+    L.tam_batch_destroy.restype = None
+    L.tam_batch_last_error.argtypes = [vp]
+    L.tam_batch_last_error.restype = C.c_char_p
+    L.tam_batch_mc_run.argtypes = [vp, C.POINTER(C.c_int64), C.c_uint64, C.c_uint32, C.c_int32, C.POINTER(MCStats)]
+    L.tam_batch_heat_run.argtypes = [vp, C.c_int32]
+    L.tam_batch_set_launch.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+    L.tam_batch_tune.argtypes = [vp, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.tam_batch_last_ms.argtypes = [vp]
+    L.tam_batch_last_ms.restype = C.c_double
     L.tam_struct_size.argtypes = [C.c_int32]
     L.tam_struct_size.restype = C.c_int64
     L.tam_pulse_advance.argtypes = [C.POINTER(Pulse), C.c_double]
@@ -292,3 +305,57 @@ class TissueGrid:
         t, k, ms = C.c_int32(), C.c_int32(), C.c_double()
         self._chk(self.L.tam_tune_heat(self.h, int(reps), C.byref(t), C.byref(k), C.byref(ms)))
         return t.value, k.value, ms.value
+
+
+class Batch:
+    """Independent TissueGrids run together: one transport launch for all,
+    one stencil launch per step for all (tam_batch_*)."""
+
+    def __init__(self, grids):
+        self.L = load()
+        self.grids = list(grids)
+        arr = (C.c_void_p * len(self.grids))(*[g.h for g in self.grids])
+        h = C.c_void_p()
+        rc = self.L.tam_batch_create(arr, len(self.grids), C.byref(h))
+        if rc != TAM_OK:
+            raise TamError(rc, self.L.tam_batch_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, rc):
+        if rc != TAM_OK:
+            raise TamError(rc, self.L.tam_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.tam_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def mc_run(self, npackets, seed, stream0=0, accumulate=False):
+        n = len(self.grids)
+        npk = (C.c_int64 * n)(*[int(x) for x in npackets])
+        st = (MCStats * n)()
+        self._chk(self.L.tam_batch_mc_run(self.h, npk, int(seed), int(stream0), int(bool(accumulate)), st))
+        return list(st)
+
+    def heat_run(self, loops):
+        self._chk(self.L.tam_batch_heat_run(self.h, int(loops)))
+
+    def set_launch(self, tile, kz, blocks_per_cu, rng_batch):
+        self._chk(self.L.tam_batch_set_launch(self.h, int(tile), int(kz), int(blocks_per_cu), int(rng_batch)))
+
+    def tune(self, sample_packets, heat_reps=5):
+        n = len(self.grids)
+        npk = (C.c_int64 * n)(*[int(x) for x in sample_packets])
+        t, k, b, r = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        self._chk(self.L.tam_batch_tune(self.h, npk, int(heat_reps), C.byref(t), C.byref(k), C.byref(b),
+                                        C.byref(r)))
+        return dict(tile=t.value, kz=k.value, blocks_per_cu=b.value, rng_batch=r.value)
+
+    def last_ms(self):
+        return self.L.tam_batch_last_ms(self.h)
