@@ -1197,7 +1197,7 @@ int tam_tune_heat(tam_ctx* c, int32_t reps, int32_t* tile, int32_t* kz, double* 
         a.eblk = eb;
         a.Q = qs;
         const int boil = c->p.boil != 0, robin = c->p.robin_top != 0, dmg = c->p.damage_every_step != 0;
-        const int kzs[] = {4, 8, 16, 32, 64};
+        const int kzs[] = {4, 8, 12, 16, 32, 64};
         for (int t = 0; t < heat_num_tiles() && e == hipSuccess; ++t)
             for (int kk : kzs) {
                 if (kk > c->p.nz && kk != 4) continue;

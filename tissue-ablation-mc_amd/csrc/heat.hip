@@ -35,6 +35,25 @@ __device__ __forceinline__ double wave_sum_d(double v)
 
 enum : unsigned { F_BOIL = 1u, F_ROBIN = 2u, F_DAMAGE = 4u, F_ENERGY = 8u };
 
+// The face sum and the update in one fixed evaluation order (explicit FMAs):
+// left to the compiler, a*b + c*d contracts to fma(a, b, c*d) or to
+// fma(c, d, a*b) depending on the surrounding code, and every kernel form and
+// launch shape must round the same way.
+__device__ __forceinline__ float face_sum(float tc, float gxm, float txm, float gxp, float txp, float gym,
+                                          float tym, float gyp, float typ, float gzm, float tm, float gzp, float tp)
+{
+    float s = gxm * (txm - tc);
+    s = fmaf(gxp, txp - tc, s);
+    s = fmaf(gym, tym - tc, s);
+    s = fmaf(gyp, typ - tc, s);
+    s = fmaf(gzm, tm - tc, s);
+    return fmaf(gzp, tp - tc, s);
+}
+__device__ __forceinline__ float step_update(float tc, float inc, float lon, float src)
+{
+    return fmaf(lon, src, tc + inc);
+}
+
 // Workgroups go to the 8 XCDs round-robin by linear id, and each XCD has its
 // own L2.  Tiles that are neighbours share halo cache lines, so the linear
 // block id is remapped to give every XCD one contiguous range of tiles
@@ -45,17 +64,20 @@ struct TileId {
     int x, y, z;
     long id;
 };
+// 32-bit arithmetic (64-bit divides cost a few hundred scalar instructions
+// per wave): every launch here has fewer than 2^32 blocks.
 __device__ __forceinline__ TileId xcd_tile()
 {
-    const long gx = gridDim.x, gy = gridDim.y;
-    const long nb = gx * gy * gridDim.z;
-    const long w = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-    const long q = nb / 8, r = nb % 8, x = w % 8, slot = w / 8;
-    const long t = x * q + (x < r ? x : r) + slot;
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned nb = gx * gy * gridDim.z;
+    const unsigned w = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned q = nb >> 3, r = nb & 7u, x = w & 7u, slot = w >> 3;
+    const unsigned t = x * q + (x < r ? x : r) + slot;
+    const unsigned txy = t / gx;
     TileId tl;
-    tl.x = (int)(t % gx);
-    tl.y = (int)((t / gx) % gy);
-    tl.z = (int)(t / (gx * gy));
+    tl.x = (int)(t - txy * gx);
+    tl.y = (int)(txy % gy);
+    tl.z = (int)(txy / gy);
     tl.id = t;
     return tl;
 }
@@ -220,11 +242,10 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
             float2 rb;
             if (LUT) rb = sRob[cc_k];
             else rb = a.robin[(i - 1) + (long)nx * (j - 1)];
-            tp_k = rb.x * tc_k + rb.y;
+            tp_k = fmaf(rb.x, tc_k, rb.y);
         }
-        const float inc = gxm * (txm - tc_k) + gxp * (txp - tc_k) + gym * (tym - tc_k) +
-                          gyp * (typ - tc_k) + gzm_f * (tm_k - tc_k) + gzp_f * (tp_k - tc_k);
-        float tn = tc_k + inc + lon * s_k;
+        const float inc = face_sum(tc_k, gxm, txm, gxp, txp, gym, tym, gyp, typ, gzm_f, tm_k, gzp_f, tp_k);
+        float tn = step_update(tc_k, inc, lon, s_k);
         if (tc_k < a.dT_neg) a.flags[0] = 1;                    // 3dFD.f90:179-182
         if (boil && tc_k >= a.dT_boil_test) {                    // 3dFD.f90:167-172
             const double T64 = a.t_ref + (double)tc_k;
@@ -357,11 +378,10 @@ __device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __re
             float2 rb;
             if (LUT) rb = sRob[cc];
             else rb = a.robin[(ic - 1) + (long)nx * (jc - 1)];
-            tp = rb.x * tc + rb.y;
+            tp = fmaf(rb.x, tc, rb.y);
         }
-        const float inc = gxm * (txm - tc) + gxp * (txp - tc) + gym * (tym - tc) + gyp * (typ - tc) +
-                          gzm * (tm - tc) + gzp * (tp - tc);
-        float tn = tc + inc + lon * s_k;
+        const float inc = face_sum(tc, gxm, txm, gxp, txp, gym, tym, gyp, typ, gzm, tm, gzp, tp);
+        float tn = step_update(tc, inc, lon, s_k);
         if (in) {
             if (tc < a.dT_neg) a.flags[0] = 1;                   // 3dFD.f90:179-182
             if (boil && tc >= a.dT_boil_test) {                   // 3dFD.f90:167-172
