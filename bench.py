@@ -140,7 +140,9 @@ def main():
     stencil["measured_copy_gbps"] = copy_gbps
     stencil["traffic"] = measured_traffic(n)
     # transport: bound by the global FP64 atomic rate of its tally (one add per
-    # voxel visit), whose ceiling is measured on this device
+    # voxel visit), whose ceiling is measured on this device.  tally_adds also
+    # counts the deposits that land in the LDS tally box (fresh tissue near the
+    # beam axis: ~1% of this workload's deposits, all in its first cycle)
     atom_peak = run.grid.atomic_rate()
     atom = T["tally_adds"] / mc_s
     mc_ops = (MC_OPS_CROSS * T["crossings"] + MC_OPS_INTER * T["interactions"]) / mc_s

@@ -110,8 +110,9 @@ typedef struct {
     double crossings, interactions;                  /* work counters */
     double kernel_ms;                                /* device time of the transport kernel */
     double killed;                                   /* abandoned after 2^24 events (exit guarantee) */
-    double tally_adds;                               /* FP64 atomic adds into the tally (one per
-                                                        voxel visit with mu_t > 0) */
+    double tally_adds;                               /* FP64 path deposits into the tally, one per
+                                                        voxel visit with mu_t > 0 (device atomics,
+                                                        or LDS atomics inside the tally box) */
 } tam_mc_stats;
 
 /* Pulse state (3dFD.f90:7-12, :199-214); advanced by tam_heat_run. */
@@ -173,9 +174,15 @@ int tam_mc_run(tam_ctx* ctx, int64_t npackets, uint64_t seed, uint32_t stream, i
  * the packets in flight; rng_batch (1..64) is how many lanes of a 64-lane
  * wave must wait for random numbers before the wave draws them together.
  * tam_tune_mc times candidate pairs on a sample of `sample` packets and keeps
- * the fastest (the tally is cleared afterwards).  Neither changes results:
- * packet p always draws from its own stream. */
+ * the fastest (the tally is cleared afterwards), with the LDS tally box on
+ * and off.  None of them changes results beyond FP64 summation order:
+ * packet p always draws from its own stream.
+ * tam_set_mc_box: each block sums the path it lays in a box of voxels on the
+ * beam axis below the top face (at most 64 KiB of FP64, sized to the
+ * blocks per CU) in LDS and adds the box to the tally once at the end,
+ * instead of one device atomic per voxel visit (1 = on, the default). */
 int tam_set_mc_launch(tam_ctx* ctx, int32_t blocks_per_cu, int32_t rng_batch);
+int tam_set_mc_box(tam_ctx* ctx, int32_t enable);
 int tam_tune_mc(tam_ctx* ctx, int64_t sample, int32_t* blocks_per_cu, int32_t* rng_batch,
                 double* packets_per_s);
 

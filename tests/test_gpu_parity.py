@@ -108,14 +108,16 @@ def test_mc_same_stream_through_a_crater():
 
 def test_mc_launch_shape_does_not_change_results():
     """Packets draw from their own streams whichever lane takes them: every
-    (blocks per CU, random-phase batch) gives the same integer fates and the
-    same tally up to FP64 summation order.  The tuner leaves the tally clear."""
+    (blocks per CU, random-phase batch, LDS tally box on / off) gives the same
+    integer fates and the same tally up to FP64 summation order.  The tuner
+    leaves the tally clear."""
     prob = _two_layer_problem(n=32, packets=300_000)
     g = prob.grid()
     res = []
-    for bpc, batch in ((1, 1), (2, 64), (8, 32), (5, 13)):
-        _print(f"case: launch bpc={bpc} batch={batch}")
+    for bpc, batch, box in ((1, 1, 1), (2, 64, 1), (8, 32, 0), (5, 13, 1), (2, 32, 0)):
+        _print(f"case: launch bpc={bpc} batch={batch} box={box}")
         g.set_mc_launch(bpc, batch)
+        g.set_mc_box(box)
         st = g.mc_run(prob.packets, seed=77, stream=3)
         res.append((st, g.download(lib.FIELD_TALLY)))
     b, t, rate = g.tune_mc(100_000)

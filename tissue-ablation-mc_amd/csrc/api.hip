@@ -78,6 +78,7 @@ McArgs mc_args(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream)
     a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32); a.stream = stream;
     a.blocks_per_cu = c->mc_bpc;
     a.rng_batch = c->mc_batch;
+    a.box_enable = c->mc_box;
     return a;
 }
 
@@ -484,6 +485,7 @@ int tam_create(const tam_params* p, tam_ctx** out)
     c->kz = 16;
     c->mc_bpc = 8;
     c->mc_batch = 32;
+    c->mc_box = 1;
     c->cyc_loops = -1;
     // every stencil tile is at least 32 x 4 and kz >= 1
     c->neblk = ((nx + 31) / 32) * ((ny + 3) / 4) * nz;
@@ -1124,29 +1126,40 @@ int tam_set_mc_launch(tam_ctx* c, int32_t blocks_per_cu, int32_t rng_batch)
     return TAM_OK;
 }
 
+int tam_set_mc_box(tam_ctx* c, int32_t enable)
+{
+    if (!c) return TAM_ERR_ARG;
+    cycle_reset(c);
+    c->mc_box = enable != 0;
+    return TAM_OK;
+}
+
 int tam_tune_mc(tam_ctx* c, int64_t sample, int32_t* blocks_per_cu, int32_t* rng_batch, double* packets_per_s)
 {
     if (!c) return TAM_ERR_ARG;
     if (sample < 1) return fail(c, TAM_ERR_ARG, "sample must be positive");
-    const int old_bpc = c->mc_bpc, old_batch = c->mc_batch;
+    const int old_bpc = c->mc_bpc, old_batch = c->mc_batch, old_box = c->mc_box;
     const int bpcs[] = {2, 4, 6, 8};
     const int batches[] = {8, 16, 24, 32, 40, 48};
     double best = 0.0;
-    int bb = c->mc_bpc, bt = c->mc_batch;
-    for (int b : bpcs) {
-        for (int t : batches) {
-            c->mc_bpc = b;
-            c->mc_batch = t;
-            tam_mc_stats st;
-            const int rc = tam_mc_run(c, sample, 0x5eed5eedULL, 0xffffffffu, 0, &st);
-            if (rc != TAM_OK) return rc;
-            const double rate = st.kernel_ms > 0 ? sample / (st.kernel_ms * 1e-3) : 0.0;
-            if (rate > best) { best = rate; bb = b; bt = t; }
+    int bb = c->mc_bpc, bt = c->mc_batch, bx = c->mc_box;
+    for (int x = 0; x < 2; ++x)
+        for (int b : bpcs) {
+            for (int t : batches) {
+                c->mc_box = x;
+                c->mc_bpc = b;
+                c->mc_batch = t;
+                tam_mc_stats st;
+                const int rc = tam_mc_run(c, sample, 0x5eed5eedULL, 0xffffffffu, 0, &st);
+                if (rc != TAM_OK) return rc;
+                const double rate = st.kernel_ms > 0 ? sample / (st.kernel_ms * 1e-3) : 0.0;
+                if (rate > best) { best = rate; bb = b; bt = t; bx = x; }
+            }
         }
-    }
     c->mc_bpc = bb;
     c->mc_batch = bt;
-    if (bb != old_bpc || bt != old_batch) cycle_reset(c);   // captured cycles use the old shape
+    c->mc_box = bx;
+    if (bb != old_bpc || bt != old_batch || bx != old_box) cycle_reset(c);   // captured cycles use the old shape
     Dev d(c->dev);
     HIPCHK(c, launch_fill_f64(c->tally, c->nvox, 0.0, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

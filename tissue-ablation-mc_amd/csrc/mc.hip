@@ -40,6 +40,9 @@
 //    distances and directions, whose distributions the parity tests check.
 #include "tam_internal.h"
 
+#include <algorithm>
+#include <cmath>
+
 namespace {
 
 __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1)
@@ -113,7 +116,25 @@ enum : int { PEND_NONE = 0, PEND_LAUNCH = 1, PEND_INTERACT = 2, PEND_FRESNEL = 3
 
 // The transport of one instance by the calling wave's lanes, until its
 // packet queue is drained; the fate counters go to a.counters.
-__device__ __forceinline__ void transport(const McArgs& a)
+// Path length into the tally: voxels inside the block's LDS box are summed
+// there (an LDS FP64 atomic), the rest go to the device tally directly.
+template <bool BOX>
+__device__ __forceinline__ void tally_add(const McArgs& a, double* box, uint32_t v, int ci, int cj, int ck, float l)
+{
+    if (BOX) {
+        const unsigned bx = (unsigned)(ci - a.box_o[0]), by = (unsigned)(cj - a.box_o[1]),
+                       bz = (unsigned)(ck - a.box_o[2]);
+        if (bx < (unsigned)a.box_n[0] && by < (unsigned)a.box_n[1] && bz < (unsigned)a.box_n[2]) {
+            __hip_atomic_fetch_add(&box[bx + (unsigned)a.box_n[0] * (by + (unsigned)a.box_n[1] * bz)], (double)l,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            return;
+        }
+    }
+    unsafeAtomicAdd(&a.tally[v], (double)l);
+}
+
+template <bool BOX>
+__device__ __forceinline__ void transport(const McArgs& a, double* box)
 {
     unsigned long long* head = a.counters + TAM_CNT_N;     // queue head, packets taken
     const uint32_t nxy = (uint32_t)a.nx * (uint32_t)a.ny;
@@ -235,7 +256,7 @@ __device__ __forceinline__ void transport(const McArgs& a)
                     ++c_inter;
                     const MatEntry me = a.mtab[mat];
                     if (!(u01(r.x) < me.albedo)) {
-                        unsafeAtomicAdd(&a.tally[v], (double)lacc);
+                        tally_add<BOX>(a, box, v, ci, cj, ck, lacc);
                         ++c_tally;
                         lacc = 0.f;
                         ++c_abs;
@@ -317,7 +338,7 @@ __device__ __forceinline__ void transport(const McArgs& a)
                 // mu_t = 0 (ablated): nothing is absorbed, and the atomic
                 // unit's throughput is what bounds this kernel
                 if (mu > 0.f) {
-                    unsafeAtomicAdd(&a.tally[v], (double)lacc);
+                    tally_add<BOX>(a, box, v, ci, cj, ck, lacc);
                     ++c_tally;
                 }
                 lacc = 0.f;
@@ -372,7 +393,30 @@ __device__ __forceinline__ void transport(const McArgs& a)
     }
 }
 
-__global__ __launch_bounds__(256) void mc_kernel(McArgs a) { transport(a); }
+__global__ __launch_bounds__(256) void mc_kernel(McArgs a) { transport<false>(a, nullptr); }
+
+// The same with the block-private LDS tally box: cleared first, added to the
+// device tally (one atomic per touched voxel) once every wave of the block
+// has drained the queue.
+__global__ __launch_bounds__(256) void mc_box_kernel(McArgs a)
+{
+    extern __shared__ double box[];
+    const int nbox = a.box_n[0] * a.box_n[1] * a.box_n[2];
+    for (int i = threadIdx.x; i < nbox; i += 256) box[i] = 0.0;
+    __syncthreads();
+    transport<true>(a, box);
+    __syncthreads();
+    const uint32_t nxy = (uint32_t)a.nx * (uint32_t)a.ny;
+    for (int i = threadIdx.x; i < nbox; i += 256) {
+        const double s = box[i];
+        if (s != 0.0) {
+            const int bx = i % a.box_n[0], by = (i / a.box_n[0]) % a.box_n[1], bz = i / (a.box_n[0] * a.box_n[1]);
+            const uint32_t v = (uint32_t)(a.box_o[0] + bx) + (uint32_t)a.nx * (uint32_t)(a.box_o[1] + by) +
+                               nxy * (uint32_t)(a.box_o[2] + bz);
+            unsafeAtomicAdd(&a.tally[v], s);
+        }
+    }
+}
 
 // Many instances in one launch: a wave takes packets from one instance's
 // queue until it drains, then moves on to the next instance (starting at its
@@ -386,7 +430,7 @@ __global__ __launch_bounds__(256) void mc_batch_kernel(const McArgs* __restrict_
         const int i = (wave + k) % n;
         const McArgs a = tab[i];
         if (a.npk == 0) continue;
-        transport(a);
+        transport<false>(a, nullptr);
     }
 }
 
@@ -439,9 +483,32 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
     // blocks_per_cu 256-lane blocks per CU (autotuned); every lane takes
     // packets from the queue until it is empty, so more blocks than packets
     // only idle
-    long blocks = (long)ncu * (a.blocks_per_cu > 0 ? a.blocks_per_cu : 8);
+    const int bpc = a.blocks_per_cu > 0 ? a.blocks_per_cu : 8;
+    long blocks = (long)ncu * bpc;
     const long need = (long)((a.npk + 255) / 256);
     if (need < blocks) blocks = need;
-    hipLaunchKernelGGL(mc_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
+    // LDS tally box: the resident blocks of a CU share its 160 KiB of LDS;
+    // at most 64 KiB (8192 voxels) per block.  An s x s x d box on the beam
+    // axis from the top face down (d ~ s / 2), where most of the path of a
+    // surface beam lies.
+    McArgs b = a;
+    b.box_n[0] = b.box_n[1] = b.box_n[2] = 0;
+    const long vox = std::min(8192L, (160L * 1024L / bpc - 1024L) / 8L);
+    int sd = (int)std::cbrt(2.0 * (double)vox);
+    sd = std::min(sd, std::min(a.nx, a.ny));
+    const int d = sd > 0 ? std::min<long>(a.nz, vox / ((long)sd * sd)) : 0;
+    if (a.box_enable && sd >= 4 && d >= 1) {
+        const int ix = (int)std::lround(a.cx / a.dx) - sd / 2, iy = (int)std::lround(a.cy / a.dy) - sd / 2;
+        b.box_o[0] = std::max(0, std::min(ix, a.nx - sd));
+        b.box_o[1] = std::max(0, std::min(iy, a.ny - sd));
+        b.box_o[2] = a.nz - d;
+        b.box_n[0] = sd;
+        b.box_n[1] = sd;
+        b.box_n[2] = d;
+        const size_t lds = sizeof(double) * (size_t)sd * (size_t)sd * (size_t)d;
+        hipLaunchKernelGGL(mc_box_kernel, dim3((unsigned)blocks), dim3(256), lds, s, b);
+    } else {
+        hipLaunchKernelGGL(mc_kernel, dim3((unsigned)blocks), dim3(256), 0, s, b);
+    }
     return hipGetLastError();
 }

@@ -50,6 +50,10 @@ struct McArgs {
     // at launch (nullptr: npk and stream above)
     const int64_t* dyn_npk;
     const uint32_t* dyn_stream;
+    int box_enable;            // use the LDS tally box (single-grid launches)
+    // block-private FP64 tally box in LDS (box_n[0] == 0: none): voxels
+    // [box_o, box_o + box_n) are summed per block and added to the tally once
+    int box_o[3], box_n[3];
 };
 
 // counters[TAM_CNT_N] is the transport's packet queue head
@@ -188,6 +192,7 @@ struct tam_ctx {
     long neblk;                 // slots: enough for the largest launch grid
     int mc_bpc;                 // transport launch: blocks per CU (autotuned)
     int mc_batch;               // transport launch: random-phase batch (autotuned)
+    int mc_box;                 // transport launch: LDS tally box on / off (autotuned)
     float2* robin;
     double *kappa, *density, *heatcap, *coeff, *rhokap;
     float* mut;
