@@ -335,6 +335,75 @@ def test_arrhenius_reference_cadence():
     assert (th[0] > 0).any() and (th[1] > 0).any()
 
 
+def test_fused_damage_constant_temperature_closed_form():
+    """FP32 path with Omega integrated inside every step: a uniform
+    temperature in the window (Dirichlet halo at the same value, no source)
+    stays put, so Omega grows by d = delt A exp(-Ea/RT) per step and the
+    threshold times follow in closed form, including the chain's one time per
+    step (0.53 and 1 are both passed in step 1; the second time is written in
+    step 2).  d is chosen so that 1e4 / d is half-way between two steps."""
+    import math
+    p = configs._config_params((12, 10, 9), (0.12, 0.10, 0.09))
+    p.robin_top = 0
+    p.boil = 0
+    p.damage_every_step = 1
+    d = p.arr_thr[2] / 200.5
+    T_star = p.arr_dE / (p.arr_R * (math.log(p.arr_A) + math.log(p.delt) - math.log(d)))
+    assert p.arr_Tlo < T_star < p.arr_Thi
+    mat = np.zeros((9, 10, 12), np.uint8)
+    temp, kappa, density, heatcap = configs._thermal_cube(p, configs.C0_THERMAL, t0=T_star)
+    prob = configs.Problem("uniform-window", p, mat, [(0.5, 100.0, 0.9, 1.38)], temp, kappa, density, heatcap)
+    g = prob.grid()
+    steps = 205
+    g.heat_run(steps)
+    T = g.download(lib.FIELD_TEMP)
+    om = g.download(lib.FIELD_TISSUE)
+    th = g.download(lib.FIELD_THRESTIME)
+    g.close()
+    T_dev = T[1:-1, 1:-1, 1:-1]
+    assert np.ptp(T_dev) == 0.0                           # uniform and unchanged
+    d_dev = p.delt * p.arr_A * np.exp(-p.arr_dE / (p.arr_R * T_dev[0, 0, 0]))
+    np.testing.assert_allclose(om, steps * d_dev, rtol=2e-5)
+    t = np.cumsum(np.full(steps, p.delt))                 # time after each step
+    np.testing.assert_allclose(th[0], t[0], rtol=1e-12)
+    np.testing.assert_allclose(th[1], t[1], rtol=1e-12)
+    np.testing.assert_allclose(th[2], t[200], rtol=1e-12)
+
+
+def test_arrhenius_chain_with_uploaded_threshold_times():
+    """The threshold chain's "ThresTime(k) == 0" tests come from a per-voxel
+    mask rebuilt on upload: uploaded times with gaps (k = 1 set, 2 clear, 3
+    set, and every other pattern) must continue exactly as the oracle's chain
+    on the FP64 times."""
+    prob = configs.reference_default(n=16)
+    p = prob.params
+    rng = np.random.default_rng(11)
+    T = prob.temp.copy()
+    T[1:-1, 1:-1, 1:-1] = 340 + 30 * rng.random((16, 16, 16))
+    th0 = np.zeros((3, 16, 16, 16))
+    pattern = rng.integers(0, 8, size=(16, 16, 16))
+    for k in range(3):
+        th0[k][(pattern >> k) & 1 == 1] = 0.25 * (k + 1)
+    om0 = 2e4 * rng.random((16, 16, 16)) ** 4            # every threshold reachable
+    g = prob.grid()
+    g.upload(lib.FIELD_TEMP, T)
+    g.upload(lib.FIELD_TISSUE, om0)
+    g.upload(lib.FIELD_THRESTIME, th0)
+    Tdev = g.download(lib.FIELD_TEMP)
+    for it in range(4):
+        g.arrhenius(0.2, 1.0 + it)
+    om_g = g.download(lib.FIELD_TISSUE)
+    th_g = g.download(lib.FIELD_THRESTIME)
+    g.close()
+    om, th = om0.copy(), th0.copy()
+    for it in range(4):
+        O.lib().or_arrhenius(C.byref(_arrh(p)), 16, 16, 16, np.ascontiguousarray(Tdev),
+                             np.ascontiguousarray(prob.rhokap), 0.2, 1.0 + it, om, th)
+    assert np.allclose(om_g, om, rtol=1e-12, atol=0)
+    assert np.array_equal(th_g, th)
+    assert (th != th0).sum() > 100                        # the chain fired, including past gaps
+
+
 def _rebuild_oracle(prob, Tdev, rhokap, mode, Q=None, W=None):
     p = prob.params
     nz, ny, nx = p.nz, p.ny, p.nx

@@ -94,7 +94,7 @@ HeatArgs heat_args(tam_ctx* c)
     a.qvapor = c->p.qvapor;
     a.dT_boil = (float)(373.0 - c->p.t_ref);
     a.t_ref = c->p.t_ref;
-    a.omega = c->omega; a.thres = c->thres; a.rhokap = c->rhokap;
+    a.omega = c->omega; a.thres = c->thres; a.tmask = c->tmask; a.rhokap = c->rhokap;
     a.arr_A = c->p.arr_A; a.arr_dE = c->p.arr_dE; a.arr_R = c->p.arr_R;
     a.arr_Tlo = c->p.arr_Tlo; a.arr_Thi = c->p.arr_Thi;
     a.thr0 = c->p.arr_thr[0]; a.thr1 = c->p.arr_thr[1]; a.thr2 = c->p.arr_thr[2];
@@ -213,7 +213,7 @@ Heat64Args heat64_args(tam_ctx* c)
     a.kappa = c->kappa; a.heatcap = c->heatcap; a.coeff = c->coeff; a.jmean = c->jmean; a.Q = c->Q;
     a.delt = c->p.delt; a.volume_voxel = c->p.volume_voxel; a.mass_voxel = c->p.mass_voxel;
     a.qvapor = c->p.qvapor; a.h_top = c->p.h_top; a.t_amb = c->p.t_amb; a.dz = c->p.dz_m;
-    a.omega = c->omega; a.thres = c->thres;
+    a.omega = c->omega; a.thres = c->thres; a.tmask = c->tmask;
     a.arr_A = c->p.arr_A; a.arr_dE = c->p.arr_dE; a.arr_R = c->p.arr_R;
     a.arr_Tlo = c->p.arr_Tlo; a.arr_Thi = c->p.arr_Thi;
     a.thr0 = c->p.arr_thr[0]; a.thr1 = c->p.arr_thr[1]; a.thr2 = c->p.arr_thr[2];
@@ -498,6 +498,7 @@ int tam_create(const tam_params* p, tam_ctx** out)
     if (e == hipSuccess) e = dalloc(&c->W, V);
     if (e == hipSuccess) e = dalloc(&c->omega, V);
     if (e == hipSuccess) e = dalloc(&c->thres, 3 * V);
+    if (e == hipSuccess) e = dalloc(&c->tmask, V);
     if (e == hipSuccess) e = dalloc(&c->energy, 4);
     if (e == hipSuccess) e = dalloc(&c->flags, 4);
     if (e == hipSuccess) e = dalloc(&c->counters, TAM_CNT_SLOTS);
@@ -528,6 +529,7 @@ int tam_create(const tam_params* p, tam_ctx** out)
     launch_fill_f64(c->W, V, 0.75, s);
     launch_fill_f64(c->omega, V, 0.0, s);
     launch_fill_f64(c->thres, 3 * V, 0.0, s);
+    launch_tmask(c->thres, c->tmask, V, s);
     launch_fill_f64(c->energy, 4, 0.0, s);
     launch_fill_f64(c->eblk, c->neblk, 0.0, s);
     launch_fill_f32((float*)c->flags, 4, 0.f, s);
@@ -555,7 +557,7 @@ void tam_destroy(tam_ctx* c)
     if (c->stream) hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->dT0, c->dT1, c->gx, c->gy, c->gz, c->src, c->robin, c->kappa, c->density,
                     c->heatcap, c->coeff, c->rhokap, c->mut, c->mat, c->mask, c->mtab, c->mua_frac,
-                    c->tally, c->jmean, c->Q, c->W, c->omega, c->thres, c->energy, c->flags,
+                    c->tally, c->jmean, c->Q, c->W, c->omega, c->thres, c->tmask, c->energy, c->flags,
                     c->counters, c->T64a, c->T64b, c->ax, c->ay, c->az, c->cls, c->glut,
                     c->robin_lut, c->eblk};
     for (void* q : ptrs)
@@ -1030,6 +1032,7 @@ int tam_upload(tam_ctx* c, int32_t field, const double* in)
     default: return fail(c, TAM_ERR_ARG, "field cannot be uploaded");
     }
     HIPCHK(c, hipMemcpyAsync(dst, in, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    if (field == TAM_FIELD_THRESTIME) HIPCHK(c, launch_tmask(c->thres, c->tmask, V, c->stream));
     if (field == TAM_FIELD_JMEAN || field == TAM_FIELD_COEFF) {
         if (refresh_src(c) != TAM_OK) return TAM_ERR_HIP;
     }
@@ -1195,18 +1198,22 @@ int tam_tune_heat(tam_ctx* c, int32_t reps, int32_t* tile, int32_t* kz, double* 
         HeatArgs a = heat_args(c);
         const long V = c->nvox;
         double *om = nullptr, *th = nullptr, *eb = nullptr, *qs = nullptr;
+        uint8_t* tm = nullptr;
         HIPCHK(c, hipMalloc((void**)&om, sizeof(double) * (size_t)V));
         hipError_t e = hipMalloc((void**)&th, sizeof(double) * 3 * (size_t)V);
         if (e == hipSuccess) e = hipMalloc((void**)&eb, sizeof(double) * (size_t)c->neblk);
         if (e == hipSuccess) e = hipMalloc((void**)&qs, sizeof(double) * (size_t)V);
+        if (e == hipSuccess) e = hipMalloc((void**)&tm, (size_t)V);
         if (e == hipSuccess) e = hipMemcpyAsync(om, c->omega, sizeof(double) * (size_t)V, hipMemcpyDeviceToDevice,
                                                 c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(qs, c->Q, sizeof(double) * (size_t)V, hipMemcpyDeviceToDevice,
                                                 c->stream);
         if (e == hipSuccess) e = launch_fill_f64(th, 3 * V, 0.0, c->stream);
+        if (e == hipSuccess) e = launch_tmask(th, tm, V, c->stream);
         if (e == hipSuccess) e = launch_fill_f64(eb, c->neblk, 0.0, c->stream);
         a.omega = om;
         a.thres = th;
+        a.tmask = tm;
         a.eblk = eb;
         a.Q = qs;
         const int boil = c->p.boil != 0, robin = c->p.robin_top != 0, dmg = c->p.damage_every_step != 0;
@@ -1231,6 +1238,7 @@ int tam_tune_heat(tam_ctx* c, int32_t reps, int32_t* tile, int32_t* kz, double* 
         (void)hipFree(th);
         (void)hipFree(eb);
         (void)hipFree(qs);
+        (void)hipFree(tm);
         if (e != hipSuccess) return hip_fail(c, "heat tuning", e);
         if (bt != c->tile || bk != c->kz) cycle_reset(c);     // captured cycles use the old shape
         c->tile = bt;

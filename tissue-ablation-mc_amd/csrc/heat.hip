@@ -166,7 +166,11 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
     const float wlo = a.dT_lo_test - a.dT_win_pad, whi = a.dT_hi_test + a.dT_win_pad;
     bool pom = damage && in && tc >= wlo && tc < whi;
     double om_pf = 0.0;
-    if (pom) om_pf = a.omega[v];
+    unsigned tm_pf = 0;
+    if (pom) {
+        om_pf = a.omega[v];
+        tm_pf = a.tmask[v];
+    }
 
     for (int k = k0; k < k1; ++k, c += pxy, v += nxy) {
         __syncthreads();                       // previous plane's LDS reads are done
@@ -192,6 +196,7 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
         const float gzm_k = gzm, gzc_k = gzc;
         const bool pom_k = pom;
         const double om_k = om_pf;
+        const unsigned tmask_k = tm_pf;
         // ---- prefetch plane k+1 (addresses stay inside the halo array: k+2 <= nz+1
         //      whenever k+1 < k1 <= nz+1; the last iteration skips it)
         if (k + 1 < k1) {
@@ -215,7 +220,10 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
             }
             s = srca[n];
             pom = damage && in && tc >= wlo && tc < whi;      // tc now holds plane k+1
-            if (pom) om_pf = a.omega[v + nxy];
+            if (pom) {
+                om_pf = a.omega[v + nxy];
+                tm_pf = a.tmask[v + nxy];
+            }
         }
         __syncthreads();
         if (!in) continue;
@@ -272,11 +280,9 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
                 const double om = (pom_k ? om_k : a.omega[v]) + a.delt * (double)expf(arg);
                 a.omega[v] = om;
                 // the threshold chain can only fire once Omega reaches the lowest threshold
-                if (om >= a.thr_min) {
-                    if (a.thres[v] == 0.0 && om >= a.thr0) a.thres[v] = t_after;
-                    else if (a.thres[v + nv] == 0.0 && om >= a.thr1) a.thres[v + nv] = t_after;
-                    else if (a.thres[v + 2 * nv] == 0.0 && om >= a.thr2) a.thres[v + 2 * nv] = t_after;
-                }
+                if (om >= a.thr_min)
+                    thres_chain(a.thres, a.tmask, v, nv, om, a.thr0, a.thr1, a.thr2, t_after,
+                                pom_k ? tmask_k : a.tmask[v]);
             }
         }
         if (energy && (bnd_ij || k == 1 || k == nz)) {
@@ -358,7 +364,11 @@ __device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __re
         // Omega issued with the plane's loads when T is already near the window
         const bool pom = damage && in && tc >= a.dT_lo_test - a.dT_win_pad && tc < a.dT_hi_test + a.dT_win_pad;
         double om_pf = 0.0;
-        if (pom) om_pf = a.omega[v];
+        unsigned tm_pf = 0;
+        if (pom) {
+            om_pf = a.omega[v];
+            tm_pf = a.tmask[v];
+        }
         float gxm, gxp, gym, gyp, gzp;
         int cp = 0;
         if (LUT) {
@@ -404,11 +414,9 @@ __device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __re
                     const float arg = (float)(a.lnA - a.arr_dE / (a.arr_R * T));
                     const double om = (pom ? om_pf : a.omega[v]) + a.delt * (double)expf(arg);
                     a.omega[v] = om;
-                    if (om >= a.thr_min) {
-                        if (a.thres[v] == 0.0 && om >= a.thr0) a.thres[v] = t_after;
-                        else if (a.thres[v + nv] == 0.0 && om >= a.thr1) a.thres[v + nv] = t_after;
-                        else if (a.thres[v + 2 * nv] == 0.0 && om >= a.thr2) a.thres[v + 2 * nv] = t_after;
-                    }
+                    if (om >= a.thr_min)
+                        thres_chain(a.thres, a.tmask, v, nv, om, a.thr0, a.thr1, a.thr2, t_after,
+                                    pom ? tm_pf : a.tmask[v]);
                 }
             }
             if (energy && (bnd_ij || k == 1 || k == nz)) {
@@ -508,9 +516,7 @@ __global__ __launch_bounds__(256) void arrhenius_kernel(HeatArgs a, const float*
         om = om + delt * a.arr_A * exp(-a.arr_dE / (a.arr_R * t));
         a.omega[v] = om;
     }
-    if (a.thres[v] == 0.0 && om >= a.thr0) a.thres[v] = time;
-    else if (a.thres[v + nv] == 0.0 && om >= a.thr1) a.thres[v + nv] = time;
-    else if (a.thres[v + 2 * nv] == 0.0 && om >= a.thr2) a.thres[v + 2 * nv] = time;
+    thres_chain(a.thres, a.tmask, v, nv, om, a.thr0, a.thr1, a.thr2, time, a.tmask[v]);
 }
 
 template <int TX, int TY, bool LUT, bool DIRECT>

@@ -78,9 +78,7 @@ __global__ __launch_bounds__(256) void heat64_kernel(Heat64Args a, const double*
             if (tn >= a.arr_Tlo && tn < a.arr_Thi) {
                 const double om = a.omega[v] + a.delt * a.arr_A * exp(-a.arr_dE / (a.arr_R * tn));
                 a.omega[v] = om;
-                if (a.thres[v] == 0.0 && om >= a.thr0) a.thres[v] = time_after;
-                else if (a.thres[v + nv] == 0.0 && om >= a.thr1) a.thres[v + nv] = time_after;
-                else if (a.thres[v + 2 * nv] == 0.0 && om >= a.thr2) a.thres[v + 2 * nv] = time_after;
+                thres_chain(a.thres, a.tmask, v, nv, om, a.thr0, a.thr1, a.thr2, time_after, a.tmask[v]);
             }
         }
         if (ENERGY && (i == 1 || i == a.nx || j == 1 || j == a.ny || k == 1 || k == a.nz)) {

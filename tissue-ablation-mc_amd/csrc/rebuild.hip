@@ -287,6 +287,24 @@ hipError_t launch_energy(int nx, int ny, int nz, long px, long pxy, const float*
                        dT, T64, t_ref, heatcap, mass_voxel, out);
     return hipGetLastError();
 }
+namespace {
+__global__ __launch_bounds__(256) void tmask_kernel(const double* __restrict__ thres, uint8_t* __restrict__ tmask,
+                                                    long nvox)
+{
+    const long v = (long)blockIdx.x * 256 + threadIdx.x;
+    if (v >= nvox) return;
+    tmask[v] = (uint8_t)((thres[v] != 0.0 ? 1u : 0u) | (thres[v + nvox] != 0.0 ? 2u : 0u) |
+                         (thres[v + 2 * nvox] != 0.0 ? 4u : 0u));
+}
+}  // namespace
+
+hipError_t launch_tmask(const double* thres, uint8_t* tmask, long nvox, hipStream_t s)
+{
+    if (nvox <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tmask_kernel, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, s, thres, tmask, nvox);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_f64(double* p, long n, double v, hipStream_t s)
 {
     hipLaunchKernelGGL(fill_f64_kernel, dim3(grid_fill(n)), dim3(256), 0, s, p, n, v);

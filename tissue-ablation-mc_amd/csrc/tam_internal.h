@@ -7,7 +7,8 @@
 //                density, heatcap, coeff, rhokap
 //   interior     nx*ny*nz: FP32 mu_t, uint8 material id, FP64 path-length
 //                tally, FP64 jmean, FP64 Q, FP64 water, FP64 Omega,
-//                FP64 threshold times (x3)
+//                FP64 threshold times (x3), uint8 mask of the threshold
+//                times already set (bit k: time k != 0)
 // Temperatures live on the device as dT = T - t_ref in FP32: the explicit
 // update adds increments far below the FP32 ulp of an absolute Kelvin value,
 // while relative to t_ref they keep ~1e-7 relative precision.
@@ -20,6 +21,25 @@
 #include "../../include/tam.h"
 
 #define TAM_MAX_MATERIALS 255
+
+// First-crossing times of the damage thresholds (3dFD.f90:447-464): the
+// reference's chain "if (ThresTime(1) == 0 .and. Omega >= thr1) ... else if
+// (ThresTime(2) == 0 ...) ... else if (ThresTime(3) == 0 ...)", with each
+// "ThresTime(k) == 0" read from bit k of the voxel's mask m instead of from
+// the FP64 time, so a step reads one byte rather than up to three doubles.
+// A time written as 0.0 leaves its bit clear, as ThresTime(k) stays 0 then.
+__device__ __forceinline__ void thres_chain(double* thres, uint8_t* tmask, long v, long nv, double om, double thr0,
+                                            double thr1, double thr2, double t, unsigned m)
+{
+    int k = -1;
+    if (!(m & 1u) && om >= thr0) k = 0;
+    else if (!(m & 2u) && om >= thr1) k = 1;
+    else if (!(m & 4u) && om >= thr2) k = 2;
+    if (k >= 0) {
+        thres[v + k * nv] = t;
+        if (t != 0.0) tmask[v] = (uint8_t)(m | (1u << k));
+    }
+}
 
 struct MatEntry {          // per material, in device memory
     float albedo, g, n, mua;
@@ -78,6 +98,7 @@ struct HeatArgs {
     // Arrhenius inside the step
     double* omega;             // interior
     double* thres;             // interior x3
+    uint8_t* tmask;            // interior: bit k set when thres[k] != 0
     const double* rhokap;      // halo
     double arr_A, arr_dE, arr_R, arr_Tlo, arr_Thi, thr0, thr1, thr2;
     double lnA;                // log(arr_A), for the in-step damage rate
@@ -124,6 +145,7 @@ struct Heat64Args {            // FP64 reference-order step (heat64.hip)
     double delt, volume_voxel, mass_voxel, qvapor;
     double h_top, t_amb, dz;
     double* omega; double* thres;
+    uint8_t* tmask;
     double arr_A, arr_dE, arr_R, arr_Tlo, arr_Thi, thr0, thr1, thr2;
     double* energy;
     int* flags;
@@ -200,6 +222,7 @@ struct tam_ctx {
     MatEntry* mtab;
     double* mua_frac;           // per material mu_a / mu_t (FP64)
     double *tally, *jmean, *Q, *W, *omega, *thres;
+    uint8_t* tmask;             // bit k: thres[k] != 0 (kept with every write of thres)
     double* energy;             // [0] boundary outflow, [1..] reduction scratch
     int* flags;
     unsigned long long* counters;
@@ -267,6 +290,8 @@ hipError_t launch_normalise(int nx, int ny, int nz, long px, long pxy, const dou
                             const CycleHdr* dyn, double scale, const double* coeff, double* jmean,
                             float* src, hipStream_t s);
 hipError_t launch_fill_f64(double* p, long n, double v, hipStream_t s);
+// tmask from the threshold times: bit k = (thres[k * nvox + v] != 0)
+hipError_t launch_tmask(const double* thres, uint8_t* tmask, long nvox, hipStream_t s);
 hipError_t launch_fill_f32(float* p, long n, float v, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, long n, hipStream_t s);
 hipError_t launch_copy(const void* in, void* out, long bytes, hipStream_t s);
