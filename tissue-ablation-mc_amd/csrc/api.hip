@@ -165,6 +165,7 @@ int classify(tam_ctx* c, const double* K, const double* D, const double* Hc, con
         rob[m] = make_float2((float)beta, (float)(beta * c->p.t_ref + gam - c->p.t_ref));
     }
     HIPCHK(c, hipMemcpyAsync(c->cls, cls.data(), (size_t)H, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_class_flags(c->cls, c->p.nx, c->p.ny, c->p.nz, c->px, c->pxy, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->glut, lut.data(), sizeof(float) * lut.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->robin_lut, rob.data(), sizeof(float2) * rob.size(), hipMemcpyHostToDevice,
                              c->stream));
@@ -236,7 +237,11 @@ hipError_t build_coeffs(tam_ctx* c)
         a.robin = c->p.robin_top; a.h_top = c->p.h_top; a.t_amb = c->p.t_amb; a.t_ref = c->p.t_ref;
         a.dz = c->p.dz_m; a.robin_coef = c->robin;
         a.rhokap = c->rhokap; a.removed_inert = c->p.removed_inert;
-        return launch_coeffs(a, c->stream);
+        hipError_t e = launch_coeffs(a, c->stream);
+        // removals change classes: refresh the homogeneous-neighbourhood bits
+        if (e == hipSuccess && c->lut) e = launch_class_flags(c->cls, c->p.nx, c->p.ny, c->p.nz, c->px, c->pxy,
+                                                              c->stream);
+        return e;
     }
     Coeff64Args a{};
     a.nx = c->p.nx; a.ny = c->p.ny; a.nz = c->p.nz;

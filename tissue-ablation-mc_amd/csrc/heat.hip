@@ -133,7 +133,7 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
     uint8_t cm = 0, cc = 0, cp = 0;
     float gzm = 0.f, gzc = 0.f;
     if (LUT) {
-        cm = cla[c - pxy]; cc = cla[c]; cp = cla[c + pxy];
+        cm = cla[c - pxy] & TAM_CLASS_MASK; cc = cla[c] & TAM_CLASS_MASK; cp = cla[c + pxy] & TAM_CLASS_MASK;
     } else {
         gzm = gza[c - pxy]; gzc = gza[c];
     }
@@ -148,10 +148,10 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
     if (hym) hT2 = Tin[c - px];
     if (hyp) hT3 = Tin[c + px];
     if (LUT) {
-        if (hxm) hC0 = cla[c - 1];
-        if (hxp) hC1 = cla[c + 1];
-        if (hym) hC2 = cla[c - px];
-        if (hyp) hC3 = cla[c + px];
+        if (hxm) hC0 = cla[c - 1] & TAM_CLASS_MASK;
+        if (hxp) hC1 = cla[c + 1] & TAM_CLASS_MASK;
+        if (hym) hC2 = cla[c - px] & TAM_CLASS_MASK;
+        if (hyp) hC3 = cla[c + px] & TAM_CLASS_MASK;
     } else {
         gxc = gxa[c]; gyc = gya[c];
         if (hxm) hGx = gxa[c - 1];
@@ -207,11 +207,11 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
             if (hym) hT2 = Tin[n - px];
             if (hyp) hT3 = Tin[n + px];
             if (LUT) {
-                cm = cc; cc = cp; cp = cla[n + pxy];
-                if (hxm) hC0 = cla[n - 1];
-                if (hxp) hC1 = cla[n + 1];
-                if (hym) hC2 = cla[n - px];
-                if (hyp) hC3 = cla[n + px];
+                cm = cc; cc = cp; cp = cla[n + pxy] & TAM_CLASS_MASK;
+                if (hxm) hC0 = cla[n - 1] & TAM_CLASS_MASK;
+                if (hxp) hC1 = cla[n + 1] & TAM_CLASS_MASK;
+                if (hym) hC2 = cla[n - px] & TAM_CLASS_MASK;
+                if (hyp) hC3 = cla[n + px] & TAM_CLASS_MASK;
             } else {
                 gzm = gzc; gzc = gza[n];
                 gxc = gxa[n]; gyc = gya[n];
@@ -350,9 +350,12 @@ __device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __re
     long v = (long)(ic - 1) + (long)nx * (jc - 1) + nxy * (k0 - 1);
     float tm = Tin[c - pxy], tc = Tin[c];
     int cm = 0, cc = 0;
+    unsigned ccr = 0;                       // class byte of (i, j, k) with its TAM_CLASS_HOM bit
     float gzm = 0.f;
     if (LUT) {
-        cm = cla[c - pxy]; cc = cla[c];
+        cm = cla[c - pxy] & TAM_CLASS_MASK;
+        ccr = cla[c];
+        cc = ccr & TAM_CLASS_MASK;
     } else {
         gzm = gza[c - pxy];
     }
@@ -371,15 +374,26 @@ __device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __re
         }
         float gxm, gxp, gym, gyp, gzp;
         int cp = 0;
+        unsigned cpr = 0;
         if (LUT) {
-            cp = cla[c + pxy];
-            const int cxm = cla[c - 1], cxp = cla[c + 1], cym = cla[c - px], cyp = cla[c + px];
-            gxm = sLut[cxm * 16 + cc];
-            gxp = sLut[cc * 16 + cxp];
-            gym = sLut[256 + cym * 16 + cc];
-            gyp = sLut[256 + cc * 16 + cyp];
-            gzp = sLut[512 + cc * 16 + cp];
-            gzm = sLut[512 + cm * 16 + cc];
+            cpr = cla[c + pxy];
+            cp = cpr & TAM_CLASS_MASK;
+            if (__all((ccr & TAM_CLASS_HOM) != 0)) {
+                // every lane's voxel has six same-class neighbours: its own
+                // class's x / y / z entries, no neighbour classes to load
+                gxm = gxp = sLut[cc * 17];
+                gym = gyp = sLut[256 + cc * 17];
+                gzm = gzp = sLut[512 + cc * 17];
+            } else {
+                const int cxm = cla[c - 1] & TAM_CLASS_MASK, cxp = cla[c + 1] & TAM_CLASS_MASK;
+                const int cym = cla[c - px] & TAM_CLASS_MASK, cyp = cla[c + px] & TAM_CLASS_MASK;
+                gxm = sLut[cxm * 16 + cc];
+                gxp = sLut[cc * 16 + cxp];
+                gym = sLut[256 + cym * 16 + cc];
+                gyp = sLut[256 + cc * 16 + cyp];
+                gzp = sLut[512 + cc * 16 + cp];
+                gzm = sLut[512 + cm * 16 + cc];
+            }
         } else {
             gxm = gxa[c - 1]; gxp = gxa[c]; gym = gya[c - px]; gyp = gya[c]; gzp = gza[c];
         }
@@ -435,6 +449,7 @@ __device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __re
         if (LUT) {
             cm = cc;
             cc = cp;
+            ccr = cpr;
         } else {
             gzm = gzp;
         }

@@ -298,6 +298,34 @@ __global__ __launch_bounds__(256) void tmask_kernel(const double* __restrict__ t
 }
 }  // namespace
 
+namespace {
+__global__ __launch_bounds__(256) void class_flags_kernel(uint8_t* cls, int nx, int ny, int nz, long px, long pxy)
+{
+    const long v = (long)blockIdx.x * 256 + threadIdx.x;
+    if (v >= (long)nx * ny * nz) return;
+    const int i = (int)(v % nx) + 1;
+    const long r = v / nx;
+    const int j = (int)(r % ny) + 1, k = (int)(r / ny) + 1;
+    const long c = i + px * j + pxy * k;
+    // only bit 7 of a byte changes here, so the neighbours' classes read the
+    // same whether or not their own flag was rewritten first
+    const unsigned m = cls[c] & TAM_CLASS_MASK;
+    const bool hom = (cls[c - 1] & TAM_CLASS_MASK) == m && (cls[c + 1] & TAM_CLASS_MASK) == m &&
+                     (cls[c - px] & TAM_CLASS_MASK) == m && (cls[c + px] & TAM_CLASS_MASK) == m &&
+                     (cls[c - pxy] & TAM_CLASS_MASK) == m && (cls[c + pxy] & TAM_CLASS_MASK) == m;
+    cls[c] = (uint8_t)(m | (hom ? TAM_CLASS_HOM : 0u));
+}
+}  // namespace
+
+hipError_t launch_class_flags(uint8_t* cls, int nx, int ny, int nz, long px, long pxy, hipStream_t s)
+{
+    const long nv = (long)nx * ny * nz;
+    if (nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(class_flags_kernel, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, cls, nx, ny, nz, px,
+                       pxy);
+    return hipGetLastError();
+}
+
 hipError_t launch_tmask(const double* thres, uint8_t* tmask, long nvox, hipStream_t s)
 {
     if (nvox <= 0) return hipSuccess;

@@ -130,6 +130,11 @@ struct HeatBatchEntry {
 
 #define TAM_NCLASS 16          // LUT classes; the last one is "removed" (no conduction)
 #define TAM_CLASS_REMOVED (TAM_NCLASS - 1)
+// Bit 7 of an interior class byte: all six neighbours have the same class,
+// so the face coefficients are the class's own x / y / z entries.  Readers
+// take the class as (byte & TAM_CLASS_MASK).
+#define TAM_CLASS_HOM 0x80u
+#define TAM_CLASS_MASK 0x0fu
 
 struct Heat64Args {            // FP64 reference-order step (heat64.hip)
     int nx, ny, nz;
@@ -283,6 +288,8 @@ hipError_t launch_removal_mask(const RebuildArgs& a, hipStream_t s);
 hipError_t launch_apply_props(const RebuildArgs& a, hipStream_t s);
 hipError_t launch_water_update(const RebuildArgs& a, hipStream_t s);
 hipError_t launch_coeffs(const CoeffArgs& a, hipStream_t s);
+// TAM_CLASS_HOM bits of the interior class bytes from the current classes
+hipError_t launch_class_flags(uint8_t* cls, int nx, int ny, int nz, long px, long pxy, hipStream_t s);
 // dyn (graph replays, else nullptr): scale from dyn->scale; nothing is
 // written when dyn->mc_on == 0 (laser off: jmean keeps its last value)
 hipError_t launch_normalise(int nx, int ny, int nz, long px, long pxy, const double* tally,
