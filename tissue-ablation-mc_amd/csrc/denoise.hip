@@ -235,83 +235,114 @@ __global__ __launch_bounds__(256) void bgrad_kernel(const uint16_t* d, long nvox
     }
 }
 
-// 2x2x2 average pooling
+// The pointwise layers move 8 consecutive BF16 channels (16 bytes) per
+// thread: every channel count here is a multiple of 8, so a chunk never
+// straddles two source tensors.
+struct Bf8 {
+    uint16_t h[8];
+};
+__device__ __forceinline__ Bf8 ld8(const uint16_t* p) { return __builtin_bit_cast(Bf8, *(const uint4*)p); }
+__device__ __forceinline__ void st8(uint16_t* p, const Bf8& v) { *(uint4*)p = __builtin_bit_cast(uint4, v); }
+
+// 2x2x2 average pooling; one thread per coarse voxel and 8-channel chunk
 __global__ void pool_kernel(const uint16_t* x, int D, int H, int W, int C, uint16_t* y)
 {
-    const long n = (long)(D / 2) * (H / 2) * (W / 2) * C;
+    const int C8 = C / 8, Wc = W / 2, Hc = H / 2;
+    const long n = (long)(D / 2) * Hc * Wc * C8;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const int c = (int)(q % C);
-        long r = q / C;
-        const int xo = (int)(r % (W / 2)); r /= (W / 2);
-        const int yo = (int)(r % (H / 2));
-        const int zo = (int)(r / (H / 2));
-        float s = 0.f;
+        const int c = (int)(q % C8) * 8;
+        long r = q / C8;
+        const int xo = (int)(r % Wc); r /= Wc;
+        const int yo = (int)(r % Hc);
+        const int zo = (int)(r / Hc);
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         for (int k = 0; k < 8; ++k) {
             const int xi = 2 * xo + (k & 1), yi = 2 * yo + ((k >> 1) & 1), zi = 2 * zo + (k >> 2);
-            s += bf2f(x[(((long)zi * H + yi) * W + xi) * C + c]);
+            const Bf8 v = ld8(x + (((long)zi * H + yi) * W + xi) * C + c);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s[j] += bf2f(v.h[j]);
         }
-        y[q] = f2bf(0.125f * s);
+        Bf8 o;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) o.h[j] = f2bf(0.125f * s[j]);
+        st8(y + q * 8, o);
     }
 }
 
-// cat(upsample2(coarse), skip): fine grid D x H x W, channels C1 + C2
+// cat(upsample2(coarse), skip): fine grid D x H x W, channels C1 + C2; one
+// thread per fine voxel and 8-channel chunk
 __global__ void upcat_kernel(const uint16_t* coarse, int C1, const uint16_t* skip, int C2, int D, int H, int W,
                              uint16_t* y)
 {
-    const int C = C1 + C2;
-    const long n = (long)D * H * W * C;
+    const int C8 = (C1 + C2) / 8, C18 = C1 / 8;
+    const long n = (long)D * H * W * C8;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const int c = (int)(q % C);
-        const long m = q / C;
-        if (c < C1) {
+        const int c8 = (int)(q % C8);
+        const long m = q / C8;
+        if (c8 < C18) {
             const int x = (int)(m % W);
             const long r = m / W;
             const int yy = (int)(r % H), z = (int)(r / H);
-            y[q] = coarse[((((long)(z / 2) * (H / 2)) + yy / 2) * (W / 2) + x / 2) * C1 + c];
+            st8(y + q * 8, ld8(coarse + ((((long)(z / 2) * (H / 2)) + yy / 2) * (W / 2) + x / 2) * C1 + c8 * 8));
         } else {
-            y[q] = skip[m * C2 + (c - C1)];
+            st8(y + q * 8, ld8(skip + m * C2 + (c8 - C18) * 8));
         }
     }
 }
 
 // gradient of the upsampled part of a concatenation, summed onto the coarse
 // grid and masked by the coarse ReLU output: d[mc][c] = (sum of the 8
-// children's g[child][c]) * (a[mc][c] > 0), c < C1
+// children's g[child][c]) * (a[mc][c] > 0), c < C1; one thread per coarse
+// voxel and 8-channel chunk
 __global__ void upcat_bwd_kernel(const uint16_t* g, int C, int C1, const uint16_t* a, int D, int H, int W,
                                  uint16_t* d)
 {
-    const int Dc = D / 2, Hc = H / 2, Wc = W / 2;
-    const long n = (long)Dc * Hc * Wc * C1;
+    const int Dc = D / 2, Hc = H / 2, Wc = W / 2, C18 = C1 / 8;
+    const long n = (long)Dc * Hc * Wc * C18;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const int c = (int)(q % C1);
-        long r = q / C1;
+        const int c = (int)(q % C18) * 8;
+        long r = q / C18;
         const int xc = (int)(r % Wc); r /= Wc;
         const int yc = (int)(r % Hc);
         const int zc = (int)(r / Hc);
-        float s = 0.f;
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         for (int k = 0; k < 8; ++k) {
             const int x = 2 * xc + (k & 1), yy = 2 * yc + ((k >> 1) & 1), z = 2 * zc + (k >> 2);
-            s += bf2f(g[(((long)z * H + yy) * W + x) * C + c]);
+            const Bf8 v = ld8(g + (((long)z * H + yy) * W + x) * C + c);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s[j] += bf2f(v.h[j]);
         }
-        d[q] = bf2f(a[q]) > 0.f ? f2bf(s) : (uint16_t)0;
+        const Bf8 av = ld8(a + q * 8);
+        Bf8 o;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) o.h[j] = bf2f(av.h[j]) > 0.f ? f2bf(s[j]) : (uint16_t)0;
+        st8(d + q * 8, o);
     }
 }
 
 // gradient of a ReLU output that feeds a skip connection and a pooling:
-// d[m][c] = (g_cat[m][C1 + c] + g_pool[parent(m)][c] / 8) * (a[m][c] > 0)
+// d[m][c] = (g_cat[m][C1 + c] + g_pool[parent(m)][c] / 8) * (a[m][c] > 0);
+// one thread per fine voxel and 8-channel chunk
 __global__ void skip_pool_bwd_kernel(const uint16_t* gcat, int Ccat, int C1, const uint16_t* gpool,
                                      const uint16_t* a, int D, int H, int W, int C, uint16_t* d)
 {
-    const long n = (long)D * H * W * C;
+    const int C8 = C / 8;
+    const long n = (long)D * H * W * C8;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const int c = (int)(q % C);
-        const long m = q / C;
+        const int c = (int)(q % C8) * 8;
+        const long m = q / C8;
         const int x = (int)(m % W);
         const long r = m / W;
         const int yy = (int)(r % H), z = (int)(r / H);
         const long mp = ((long)(z / 2) * (H / 2) + yy / 2) * (W / 2) + x / 2;
-        const float s = bf2f(gcat[m * Ccat + C1 + c]) + 0.125f * bf2f(gpool[mp * C + c]);
-        d[q] = bf2f(a[q]) > 0.f ? f2bf(s) : (uint16_t)0;
+        const Bf8 gc = ld8(gcat + m * Ccat + C1 + c), gp = ld8(gpool + mp * C + c), av = ld8(a + q * 8);
+        Bf8 o;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float sj = bf2f(gc.h[j]) + 0.125f * bf2f(gp.h[j]);
+            o.h[j] = bf2f(av.h[j]) > 0.f ? f2bf(sj) : (uint16_t)0;
+        }
+        st8(d + q * 8, o);
     }
 }
 
@@ -609,19 +640,19 @@ hipError_t forward(tam_dn* n, const float* tgt)
     };
     if ((e = conv(0, n->x0, n->a[0])) != hipSuccess) return e;
     if ((e = conv(1, n->a[0], n->a[1])) != hipSuccess) return e;
-    hipLaunchKernelGGL(pool_kernel, dim3(grid_of(n->nv[1] * 16)), dim3(256), 0, n->stream, n->a[1], n->D[0], n->H[0],
+    hipLaunchKernelGGL(pool_kernel, dim3(grid_of(n->nv[1] * 16 / 8)), dim3(256), 0, n->stream, n->a[1], n->D[0], n->H[0],
                        n->W[0], 16, n->p1);
     if ((e = conv(2, n->p1, n->a[2])) != hipSuccess) return e;
     if ((e = conv(3, n->a[2], n->a[3])) != hipSuccess) return e;
-    hipLaunchKernelGGL(pool_kernel, dim3(grid_of(n->nv[2] * 32)), dim3(256), 0, n->stream, n->a[3], n->D[1], n->H[1],
+    hipLaunchKernelGGL(pool_kernel, dim3(grid_of(n->nv[2] * 32 / 8)), dim3(256), 0, n->stream, n->a[3], n->D[1], n->H[1],
                        n->W[1], 32, n->p3);
     if ((e = conv(4, n->p3, n->a[4])) != hipSuccess) return e;
     if ((e = conv(5, n->a[4], n->a[5])) != hipSuccess) return e;
-    hipLaunchKernelGGL(upcat_kernel, dim3(grid_of(n->nv[1] * 96)), dim3(256), 0, n->stream, n->a[5], 64, n->a[3], 32,
+    hipLaunchKernelGGL(upcat_kernel, dim3(grid_of(n->nv[1] * 96 / 8)), dim3(256), 0, n->stream, n->a[5], 64, n->a[3], 32,
                        n->D[1], n->H[1], n->W[1], n->c6);
     if ((e = conv(6, n->c6, n->a[6])) != hipSuccess) return e;
     if ((e = conv(7, n->a[6], n->a[7])) != hipSuccess) return e;
-    hipLaunchKernelGGL(upcat_kernel, dim3(grid_of(n->nv[0] * 48)), dim3(256), 0, n->stream, n->a[7], 32, n->a[1], 16,
+    hipLaunchKernelGGL(upcat_kernel, dim3(grid_of(n->nv[0] * 48 / 8)), dim3(256), 0, n->stream, n->a[7], 32, n->a[1], 16,
                        n->D[0], n->H[0], n->W[0], n->c8);
     if ((e = conv(8, n->c8, n->a[8])) != hipSuccess) return e;
     if ((e = conv(9, n->a[8], n->a[9])) != hipSuccess) return e;
@@ -655,25 +686,25 @@ hipError_t backward(tam_dn* n, const float* tgt)
     if ((e = dgrad(9, n->d[9], n->a[8], n->d[8])) != hipSuccess) return e;
     if ((e = grads(8, n->c8, n->d[8])) != hipSuccess) return e;
     if ((e = dgrad(8, n->d[8], nullptr, n->gc8)) != hipSuccess) return e;
-    hipLaunchKernelGGL(upcat_bwd_kernel, dim3(grid_of(n->nv[1] * 32)), dim3(256), 0, n->stream, n->gc8, 48, 32,
+    hipLaunchKernelGGL(upcat_bwd_kernel, dim3(grid_of(n->nv[1] * 32 / 8)), dim3(256), 0, n->stream, n->gc8, 48, 32,
                        n->a[7], n->D[0], n->H[0], n->W[0], n->d[7]);
     if ((e = grads(7, n->a[6], n->d[7])) != hipSuccess) return e;
     if ((e = dgrad(7, n->d[7], n->a[6], n->d[6])) != hipSuccess) return e;
     if ((e = grads(6, n->c6, n->d[6])) != hipSuccess) return e;
     if ((e = dgrad(6, n->d[6], nullptr, n->gc6)) != hipSuccess) return e;
-    hipLaunchKernelGGL(upcat_bwd_kernel, dim3(grid_of(n->nv[2] * 64)), dim3(256), 0, n->stream, n->gc6, 96, 64,
+    hipLaunchKernelGGL(upcat_bwd_kernel, dim3(grid_of(n->nv[2] * 64 / 8)), dim3(256), 0, n->stream, n->gc6, 96, 64,
                        n->a[5], n->D[1], n->H[1], n->W[1], n->d[5]);
     if ((e = grads(5, n->a[4], n->d[5])) != hipSuccess) return e;
     if ((e = dgrad(5, n->d[5], n->a[4], n->d[4])) != hipSuccess) return e;
     if ((e = grads(4, n->p3, n->d[4])) != hipSuccess) return e;
     if ((e = dgrad(4, n->d[4], nullptr, n->gp3)) != hipSuccess) return e;
-    hipLaunchKernelGGL(skip_pool_bwd_kernel, dim3(grid_of(n->nv[1] * 32)), dim3(256), 0, n->stream, n->gc6, 96, 64,
+    hipLaunchKernelGGL(skip_pool_bwd_kernel, dim3(grid_of(n->nv[1] * 32 / 8)), dim3(256), 0, n->stream, n->gc6, 96, 64,
                        n->gp3, n->a[3], n->D[1], n->H[1], n->W[1], 32, n->d[3]);
     if ((e = grads(3, n->a[2], n->d[3])) != hipSuccess) return e;
     if ((e = dgrad(3, n->d[3], n->a[2], n->d[2])) != hipSuccess) return e;
     if ((e = grads(2, n->p1, n->d[2])) != hipSuccess) return e;
     if ((e = dgrad(2, n->d[2], nullptr, n->gp1)) != hipSuccess) return e;
-    hipLaunchKernelGGL(skip_pool_bwd_kernel, dim3(grid_of(n->nv[0] * 16)), dim3(256), 0, n->stream, n->gc8, 48, 32,
+    hipLaunchKernelGGL(skip_pool_bwd_kernel, dim3(grid_of(n->nv[0] * 16 / 8)), dim3(256), 0, n->stream, n->gc8, 48, 32,
                        n->gp1, n->a[1], n->D[0], n->H[0], n->W[0], 16, n->d[1]);
     if ((e = grads(1, n->a[0], n->d[1])) != hipSuccess) return e;
     if ((e = dgrad(1, n->d[1], n->a[0], n->d[0])) != hipSuccess) return e;
