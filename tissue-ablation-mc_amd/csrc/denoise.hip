@@ -48,24 +48,65 @@ struct ConvArgs {
     int D, H, W, cin, cout, kb, relu;
 };
 
+// Raw buffer descriptor over `bytes` bytes from p (at most 2^31 - 1): loads
+// at offsets past it return zero.  0x00020000: gfx9 buffer word 3 (32-bit
+// data format, untyped access).
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ Rsrc rsrc_n(const void* p, long bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)(bytes < 0x7fffffffL ? bytes : 0x7fffffffL),
+                                             0x00020000);
+}
+__device__ __forceinline__ bf16x8 bload8(Rsrc r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+}
+
 // Forward convolution (and, with flipped weights, the data gradient).
 // A wave computes MT x 16 consecutive voxels of the flattened grid (each
-// lane finds its voxel's x, y, z) for all NT x 16 output channels.
-template <int NT, int MT>
+// lane finds its voxel's x, y, z) for all NT x 16 output channels; CIN is
+// the input channel count.
+// Addressing: the input descriptor starts HW + W + 1 voxels before the
+// block's first voxel, so every tap of the block's voxels sits at a
+// non-negative 32-bit offset; a tap outside the grid (zero padding) gets an
+// offset past the descriptor's end and loads zeros.  Each lane keeps a
+// 27-bit mask of its voxel's valid taps.  With the k loop unrolled, a
+// k-block's tap and channel for each of the four lane groups are constants,
+// and their offsets scalar values.
+template <int CIN, int NT, int MT>
 __global__ __launch_bounds__(256) void conv_kernel(ConvArgs a)
 {
-    const int lane = threadIdx.x & 63;
-    const long nvox = (long)a.D * a.H * a.W;
-    const long mbase = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * (16 * MT);
+    constexpr int KB = (27 * CIN + 31) / 32;
+    constexpr unsigned OUT = 0x80000000u;          // past every descriptor's end
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    const int W = a.W, H = a.H;
+    const long HW = (long)H * W, nvox = HW * a.D;
+    const long mblk = (long)blockIdx.x * (4 * 16 * MT);
+    const long mbase = mblk + (long)(threadIdx.x >> 6) * (16 * MT);
     if (mbase >= nvox) return;
-    int vx[MT], vy[MT], vz[MT];
+    const long back = HW + W + 1;
+    const long vb = mblk > back ? mblk - back : 0;
+    const Rsrc rx = rsrc_n(a.x + vb * CIN, (nvox - vb) * CIN * 2);
+    const Rsrc rw = rsrc_n(a.wp, (long)KB * NT * 512 * 2);
+    unsigned voff[MT], vmask[MT];
 #pragma unroll
     for (int t = 0; t < MT; ++t) {
         const long m = mbase + t * 16 + (lane & 15);
-        vx[t] = (int)(m % a.W);
-        const long r = m / a.W;
-        vy[t] = (int)(r % a.H);
-        vz[t] = (int)(r / a.H);
+        const int vx = (int)(m % W);
+        const long r = m / W;
+        const int vy = (int)(r % H), vz = (int)(r / H);
+        voff[t] = (unsigned)((m - vb) * CIN * 2);
+        const unsigned bx = (vx > 0 ? 1u : 0u) | 2u | (vx < W - 1 ? 4u : 0u);
+        const unsigned by = (vy > 0 ? 1u : 0u) | 2u | (vy < H - 1 ? 4u : 0u);
+        const unsigned bz = (vz > 0 ? 1u : 0u) | 2u | (vz < a.D - 1 ? 4u : 0u);
+        unsigned vm = 0;
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+                if ((bz >> dz) & (by >> dy) & 1u) vm |= bx << (9 * dz + 3 * dy);
+        vmask[t] = vm;
     }
     f32x4 acc[MT][NT];
 #pragma unroll
@@ -73,33 +114,34 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs a)
 #pragma unroll
         for (int n = 0; n < NT; ++n) acc[t][n] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    int tap = 0, ci = (lane >> 4) * 8;
-    while (ci >= a.cin) { ci -= a.cin; ++tap; }
-    const uint16_t* wl = a.wp + (long)lane * 8;
-    for (int kb = 0; kb < a.kb; ++kb) {
-        bf16x8 af[MT];
-        if (tap < 27) {
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+        // lane group gg holds k = 32 kb + 8 gg .. + 7: one tap, 8 channels
+        int tapg[4], offg[4];
+#pragma unroll
+        for (int gg = 0; gg < 4; ++gg) {
+            const int k = 32 * kb + 8 * gg;
+            const int tap = k / CIN, ci = k % CIN;
             const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+            tapg[gg] = tap;                           // 27 (past the last tap) never has its mask bit set
+            offg[gg] = (int)(((dz * HW + (long)dy * W + dx) * CIN + ci) * 2);
+        }
+        const int tap = g == 0 ? tapg[0] : g == 1 ? tapg[1] : g == 2 ? tapg[2] : tapg[3];
+        const int toff = g == 0 ? offg[0] : g == 1 ? offg[1] : g == 2 ? offg[2] : offg[3];
+        bf16x8 af[MT];
 #pragma unroll
-            for (int t = 0; t < MT; ++t) {
-                const int sx = vx[t] + dx, sy = vy[t] + dy, sz = vz[t] + dz;
-                const bool in = sx >= 0 && sx < a.W && sy >= 0 && sy < a.H && sz >= 0 && sz < a.D;
-                af[t] = in ? load8(a.x + (((long)sz * a.H + sy) * a.W + sx) * a.cin + ci) : zero8();
-            }
-        } else {
-#pragma unroll
-            for (int t = 0; t < MT; ++t) af[t] = zero8();
+        for (int t = 0; t < MT; ++t) {
+            const bool valid = (vmask[t] >> tap) & 1u;
+            af[t] = bload8(rx, valid ? voff[t] + (unsigned)toff : OUT, 0);
         }
         bf16x8 bfr[NT];
 #pragma unroll
-        for (int n = 0; n < NT; ++n) bfr[n] = load8(wl + ((long)kb * NT + n) * 512);
+        for (int n = 0; n < NT; ++n) bfr[n] = bload8(rw, (unsigned)lane * 16u, (unsigned)((kb * NT + n) * 1024));
 #pragma unroll
         for (int t = 0; t < MT; ++t)
 #pragma unroll
             for (int n = 0; n < NT; ++n)
                 acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[t], bfr[n], acc[t][n], 0, 0, 0);
-        ci += 32;
-        while (ci >= a.cin) { ci -= a.cin; ++tap; }
     }
     // epilogue: lane holds voxels mbase + 16 t + 4 (lane/16) + j, channel 16 n + lane%16
 #pragma unroll
@@ -578,16 +620,25 @@ hipError_t conv_launch(tam_dn* n, const uint16_t* x, int lvl, int cin, const uin
 {
     ConvArgs a{x, wp, bias, mask, y, n->D[lvl], n->H[lvl], n->W[lvl], cin, cout, kb, relu};
     const long nv = n->nv[lvl];
-    const int MT = 2;                                  // 32-voxel wave tiles: W >= 32
+    const int MT = 4;                                  // 64-voxel wave tiles (every level's nvox is a multiple)
+    if (nv % (16 * MT) != 0 || kb != (27 * cin + 31) / 32) return hipErrorInvalidValue;
     const unsigned blocks = (unsigned)((nv + 4 * 16 * MT - 1) / (4 * 16 * MT));
-    switch (cout) {
-    case 16: hipLaunchKernelGGL((conv_kernel<1, MT>), dim3(blocks), dim3(256), 0, n->stream, a); break;
-    case 32: hipLaunchKernelGGL((conv_kernel<2, MT>), dim3(blocks), dim3(256), 0, n->stream, a); break;
-    case 48: hipLaunchKernelGGL((conv_kernel<3, MT>), dim3(blocks), dim3(256), 0, n->stream, a); break;
-    case 64: hipLaunchKernelGGL((conv_kernel<4, MT>), dim3(blocks), dim3(256), 0, n->stream, a); break;
-    case 96: hipLaunchKernelGGL((conv_kernel<6, MT>), dim3(blocks), dim3(256), 0, n->stream, a); break;
+#define CV(CI, NT_) hipLaunchKernelGGL((conv_kernel<CI, NT_, MT>), dim3(blocks), dim3(256), 0, n->stream, a)
+    switch (cin * 1000 + cout) {
+    case 16016: CV(16, 1); break;      // layers 0, 1, 9 and the data gradients of 0, 1, 9
+    case 16032: CV(16, 2); break;      // layer 2
+    case 32032: CV(32, 2); break;      // layers 3, 7 and their data gradients
+    case 32064: CV(32, 4); break;      // layer 4
+    case 64064: CV(64, 4); break;      // layer 5 and its data gradient
+    case 96032: CV(96, 2); break;      // layer 6
+    case 48016: CV(48, 1); break;      // layer 8
+    case 32016: CV(32, 1); break;      // data gradient of layer 2
+    case 64032: CV(64, 2); break;      // data gradient of layer 4
+    case 32096: CV(32, 6); break;      // data gradient of layer 6
+    case 16048: CV(16, 3); break;      // data gradient of layer 8
     default: return hipErrorInvalidValue;
     }
+#undef CV
     return hipGetLastError();
 }
 
