@@ -620,22 +620,25 @@ hipError_t conv_launch(tam_dn* n, const uint16_t* x, int lvl, int cin, const uin
 {
     ConvArgs a{x, wp, bias, mask, y, n->D[lvl], n->H[lvl], n->W[lvl], cin, cout, kb, relu};
     const long nv = n->nv[lvl];
-    const int MT = 4;                                  // 64-voxel wave tiles (every level's nvox is a multiple)
-    if (nv % (16 * MT) != 0 || kb != (27 * cin + 31) / 32) return hipErrorInvalidValue;
-    const unsigned blocks = (unsigned)((nv + 4 * 16 * MT - 1) / (4 * 16 * MT));
-#define CV(CI, NT_) hipLaunchKernelGGL((conv_kernel<CI, NT_, MT>), dim3(blocks), dim3(256), 0, n->stream, a)
+    // 16 x MT-voxel wave tiles: 64 voxels, or 128 for the 48-channel input
+    // layer (its long k loop amortises the extra registers); every level's
+    // voxel count is a multiple of 512
+    if (nv % 512 != 0 || kb != (27 * cin + 31) / 32) return hipErrorInvalidValue;
+    const unsigned b8 = (unsigned)(nv / (4 * 16 * 8)), b4 = (unsigned)(nv / (4 * 16 * 4));
+#define CV(CI, NT_, MT_) \
+    hipLaunchKernelGGL((conv_kernel<CI, NT_, MT_>), dim3(MT_ == 8 ? b8 : b4), dim3(256), 0, n->stream, a)
     switch (cin * 1000 + cout) {
-    case 16016: CV(16, 1); break;      // layers 0, 1, 9 and the data gradients of 0, 1, 9
-    case 16032: CV(16, 2); break;      // layer 2
-    case 32032: CV(32, 2); break;      // layers 3, 7 and their data gradients
-    case 32064: CV(32, 4); break;      // layer 4
-    case 64064: CV(64, 4); break;      // layer 5 and its data gradient
-    case 96032: CV(96, 2); break;      // layer 6
-    case 48016: CV(48, 1); break;      // layer 8
-    case 32016: CV(32, 1); break;      // data gradient of layer 2
-    case 64032: CV(64, 2); break;      // data gradient of layer 4
-    case 32096: CV(32, 6); break;      // data gradient of layer 6
-    case 16048: CV(16, 3); break;      // data gradient of layer 8
+    case 16016: CV(16, 1, 4); break;      // layers 0, 1, 9 and the data gradients of 0, 1, 9
+    case 16032: CV(16, 2, 4); break;      // layer 2
+    case 32032: CV(32, 2, 4); break;      // layers 3, 7 and their data gradients
+    case 32064: CV(32, 4, 4); break;      // layer 4
+    case 64064: CV(64, 4, 4); break;      // layer 5 and its data gradient
+    case 96032: CV(96, 2, 4); break;      // layer 6
+    case 48016: CV(48, 1, 8); break;      // layer 8
+    case 32016: CV(32, 1, 4); break;      // data gradient of layer 2
+    case 64032: CV(64, 2, 4); break;      // data gradient of layer 4
+    case 32096: CV(32, 6, 4); break;      // data gradient of layer 6
+    case 16048: CV(16, 3, 4); break;      // data gradient of layer 8
     default: return hipErrorInvalidValue;
     }
 #undef CV
