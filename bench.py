@@ -33,17 +33,25 @@ LANE_OPS_PEAK = 256 * 64 * 2.4e9      # 256 CU x 64 lanes x 2.4 GHz (non-packed 
 STENCIL_BYTES_BASE = 13.0
 STENCIL_BYTES_DAMAGE = 16.0
 MC_OPS_CROSS, MC_OPS_INTER = 16.0, 64.0   # algorithmic FP32 lane-ops per crossing / interaction
+# heat launch shapes (tam_set_heat_launch tile ids): name, kernel, steps per launch
+SHAPES = [("lds-64x4", "stencil_kernel<64,4>", 1), ("lds-32x8", "stencil_kernel<32,8>", 1),
+          ("lds-64x8", "stencil_kernel<64,8>", 1), ("direct-64x4", "stencil_direct_kernel<64,4>", 1),
+          ("direct-32x8", "stencil_direct_kernel<32,8>", 1), ("direct-64x8", "stencil_direct_kernel<64,8>", 1),
+          ("two-step-32x8", "stencil_t2_kernel<32,8>", 2), ("two-step-32x16", "stencil_t2_kernel<32,16>", 2),
+          ("two-step-64x8", "stencil_t2_kernel<64,8>", 2)]
 
 
-def measured_traffic(n):
-    """HBM bytes per stencil launch from the committed counter summary (rocprofv3
-    --pmc FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's reading
-    for the streaming copy of known size), or None where none was recorded."""
+def measured_traffic(n, shape):
+    """HBM bytes per launch of the heat kernel of launch shape `shape` from the
+    committed counter summary (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each
+    scaled by the same counter's reading for the streaming copy of known size),
+    or None where none was recorded for that shape."""
     path = os.path.join(ROOT, "profiles", f"stencil_traffic_{n}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        rec = json.load(f).get("by_shape", {}).get(shape)
+    return rec.get("bytes_per_launch") if rec else None
 
 
 def cpu_baseline(prob, budget_s):
@@ -137,15 +145,27 @@ def main():
     value = T["packets"] / wall
     vu = T["voxel_updates"] / wall
     mc_s, heat_s = T["mc_ms"] / 1e3, T["heat_ms"] / 1e3
-    step_ms = T["heat_ms"] / max(1, T["heat_launches"])
+    # the roofline is the kernel of the launch shape that ran the most timed
+    # steps (the stencil shape is re-tuned every cycle), at that shape's time
+    (tile, kz), (shape_ms, shape_steps) = max(run.heat_by_shape.items(), key=lambda kv: kv[1][1])
+    step_ms = shape_ms / max(1, shape_steps)
+    shape, kernel, per_launch = SHAPES[tile]
+    launch_ms = step_ms * per_launch
+    # algorithmic bytes per voxel-update x the voxel-updates of one launch (a
+    # two-step launch updates every voxel twice); the traffic figure beside it
+    # is what the launch moves (the two-step kernel keeps the intermediate
+    # field on chip, so it moves less than the algorithmic bytes)
     per_vu = STENCIL_BYTES_BASE + (STENCIL_BYTES_DAMAGE if p.damage_every_step else 0.0)
-    stencil = {"bound": "hbm", "achieved": per_vu * nvox / (step_ms * 1e-3) / 1e9,
+    units = per_launch * nvox
+    stencil = {"bound": "hbm", "achieved": per_vu * units / (launch_ms * 1e-3) / 1e9,
                "peak": HBM_PEAK_GBS, "unit": "GB/s", "bytes_per_voxel_update": per_vu,
+               "voxel_updates_per_launch": units, "steps_per_launch": per_launch, "launch_us": launch_ms * 1e3,
+               "step_us": step_ms * 1e3,
                "bytes_basis": "T read+write 8, source 4, class 1" +
                               (", Omega read+write 16 (damage in every step)" if p.damage_every_step else "")}
     stencil["frac"] = stencil["achieved"] / stencil["peak"]
     stencil["measured_copy_gbps"] = copy_gbps
-    stencil["traffic"] = measured_traffic(n)
+    stencil["traffic"] = measured_traffic(n, shape)
     # transport: bound by the global FP64 atomic rate of its tally (one add per
     # voxel visit), whose ceiling is measured on this device.  tally_adds also
     # counts the deposits that land in the LDS tally box (fresh tissue near the
@@ -157,7 +177,9 @@ def main():
                "frac": atom / atom_peak, "tally_adds_per_packet": T["tally_adds"] / T["packets"],
                "valu": {"achieved": mc_ops / 1e12, "peak": LANE_OPS_PEAK / 1e12, "unit": "Tlane-op/s",
                         "frac": mc_ops / LANE_OPS_PEAK}}
-    roofline = dict(stencil, kernel="heat_step_kernel")
+    roofline = dict(stencil, kernel=f"{kernel} ({shape}, kz {kz})",
+                    steps_at_this_shape=f"{shape_steps} of {int(T['heat_steps'])}",
+                    shapes_used={f"{SHAPES[t][0]}/kz{k}": v[1] for (t, k), v in run.heat_by_shape.items()})
     line = {
         "metric": "photons/s + FD voxel-steps/s; coupled cycle time, denoised vs plain @<=2% err",
         "value": value, "unit": "photons/s", "n_gpus": 1, "steps": cycles, "warmup": args.warmup,

@@ -200,8 +200,12 @@ int tam_heat_run(tam_ctx* ctx, int32_t loops, tam_pulse* pulse);
  * split is fixed by the rank count, 3dFD.f90:45-72).  tile: 0 = 64x4,
  * 1 = 32x8, 2 = 64x8 (x by y) voxels per block with x-y planes staged in
  * LDS; 3, 4, 5 = the same tiles loading neighbours directly (no LDS, no
- * barriers); kz: z planes per block.  Every shape gives bit-identical
- * fields.
+ * barriers); 6, 7, 8 = two steps per launch on 32x8, 32x16, 64x8 footprints
+ * (the inner 30x6, 30x14, 62x6 columns owned; the intermediate field stays
+ * on chip), used in class mode without boiling, with an odd last step and
+ * every other step on direct 32x8 (6, 7) or direct 64x8 (8); kz: z planes
+ * per block.  Every shape gives bit-identical fields, damage and threshold
+ * times.
  * tam_tune_heat times every candidate on the current field (the result goes
  * to the scratch buffer, the state is not advanced) and keeps the fastest. */
 int tam_set_heat_launch(tam_ctx* ctx, int32_t tile, int32_t kz);

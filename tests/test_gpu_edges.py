@@ -44,7 +44,7 @@ def test_heat_odd_grids_all_shapes(n):
     _print(f"case: heat on {n}, 30 steps, every launch shape")
     prob, jm = _block(n, seed=sum(n))
     ref = None
-    for tile in range(6):
+    for tile in range(9):
         for kz in (1, 2, 5):
             g = prob.grid()
             g.upload(lib.FIELD_JMEAN, jm)

@@ -335,7 +335,8 @@ def test_arrhenius_reference_cadence():
     assert (th[0] > 0).any() and (th[1] > 0).any()
 
 
-def test_fused_damage_constant_temperature_closed_form():
+@pytest.mark.parametrize("shape", [None, 8])         # 8: two steps per launch
+def test_fused_damage_constant_temperature_closed_form(shape):
     """FP32 path with Omega integrated inside every step: a uniform
     temperature in the window (Dirichlet halo at the same value, no source)
     stays put, so Omega grows by d = delt A exp(-Ea/RT) per step and the
@@ -354,6 +355,8 @@ def test_fused_damage_constant_temperature_closed_form():
     temp, kappa, density, heatcap = configs._thermal_cube(p, configs.C0_THERMAL, t0=T_star)
     prob = configs.Problem("uniform-window", p, mat, [(0.5, 100.0, 0.9, 1.38)], temp, kappa, density, heatcap)
     g = prob.grid()
+    if shape is not None:
+        g.set_heat_launch(shape, 4)
     steps = 205
     g.heat_run(steps)
     T = g.download(lib.FIELD_TEMP)

@@ -80,13 +80,15 @@ def _oracle(prob, jmean, steps):
 @pytest.mark.parametrize("random_props", [False, True])
 def test_launch_shapes_bit_identical(random_props):
     prob, jm = _crater_problem(random_props=random_props)
-    ref, t0 = _gpu_run(prob, jm, 40, tile=0, kz=4)
-    _print("class mode" if t0.class_mode else "array mode")
-    assert bool(t0.class_mode) == (not random_props)
-    for tile in range(6):                       # 0-2: LDS planes, 3-5: direct neighbour loads
-        for kz in (1, 3, 8, 32):
-            T, _ = _gpu_run(prob, jm, 40, tile=tile, kz=kz)
-            assert np.array_equal(T, ref), (tile, kz)
+    for steps in (40, 41):                      # 41: a two-step run ends with one step alone
+        ref, t0 = _gpu_run(prob, jm, steps, tile=0, kz=4)
+        _print("class mode" if t0.class_mode else "array mode", steps, "steps")
+        assert bool(t0.class_mode) == (not random_props)
+        # 0-2: LDS planes, 3-5: direct neighbour loads, 6-8: two steps per launch
+        for tile in range(9):
+            for kz in (1, 3, 8, 32):
+                T, _ = _gpu_run(prob, jm, steps, tile=tile, kz=kz)
+                assert np.array_equal(T, ref), (steps, tile, kz)
 
 
 @pytest.mark.parametrize("random_props", [False, True])
@@ -139,5 +141,5 @@ def test_autotune_keeps_state():
     after = g.download(lib.FIELD_TEMP)
     g.close()
     _print("tuned", lib.__name__, tile, kz, ms)
-    assert 0 <= tile < 6 and kz >= 1 and ms > 0
+    assert 0 <= tile < 9 and kz >= 1 and ms > 0
     assert np.array_equal(before, after)

@@ -115,6 +115,15 @@ HeatArgs heat_args(tam_ctx* c)
     return a;
 }
 
+// The step loops pair steps into two-step launches when the shape is a
+// two-step one and the step allows it (FP32 class mode, no boiling: the
+// boiling clamp changes Q, which a neighbour block's redundant first step
+// would read); otherwise every step runs alone on the shape's one-step twin.
+bool two_step(const tam_ctx* c)
+{
+    return !c->fp64 && c->lut && !c->p.boil && heat_tile_steps(c->tile) == 2;
+}
+
 // Thermal classes for the FP32 stencil's class mode: every distinct
 // (kappa, density, heatcap) triple over the halo grid gets an id (at most
 // TAM_NCLASS-1); removed interior voxels get TAM_CLASS_REMOVED.  The face
@@ -340,6 +349,7 @@ int enqueue_cycle(tam_ctx* c)
     const int boil = c->p.boil != 0, robin = c->p.robin_top != 0, dmg = c->p.damage_every_step != 0;
     HeatArgs a = heat_args(c);
     Heat64Args a64 = heat64_args(c);
+    const bool two = two_step(c);
     for (int k = 0; k < c->cyc_loops; ++k) {
         if (c->fp64) {
             a64.dyn_lon = lon + k;
@@ -349,8 +359,14 @@ int enqueue_cycle(tam_ctx* c)
         } else {
             a.dyn_lon = lon + k;
             a.dyn_tafter = ta + k;
-            HIPCHK(c, launch_heat_step(a, c->cur, c->nxt, 0.f, 0.0, boil, robin, dmg, 1, c->lut ? 1 : 0, c->tile,
-                                       c->kz, s));
+            if (two && k + 1 < c->cyc_loops) {      // steps k and k+1 in one launch
+                HIPCHK(c, launch_heat_step2(a, c->cur, c->nxt, 0.f, 0.f, 0.0, 0.0, robin, dmg, 1, c->tile, c->kz,
+                                            s));
+                ++k;
+            } else {
+                HIPCHK(c, launch_heat_step(a, c->cur, c->nxt, 0.f, 0.0, boil, robin, dmg, 1, c->lut ? 1 : 0,
+                                           c->tile, c->kz, s));
+            }
             std::swap(c->cur, c->nxt);
         }
     }
@@ -492,8 +508,7 @@ int tam_create(const tam_params* p, tam_ctx** out)
     c->mc_batch = 32;
     c->mc_box = 1;
     c->cyc_loops = -1;
-    // every stencil tile is at least 32 x 4 and kz >= 1
-    c->neblk = ((nx + 31) / 32) * ((ny + 3) / 4) * nz;
+    c->neblk = heat_max_blocks(nx, ny, nz);      // one energy slot per block of any launch shape
     if (e == hipSuccess) e = dalloc(&c->eblk, c->neblk);
     if (e == hipSuccess) e = dalloc(&c->mtab, TAM_MAX_MATERIALS + 1);
     if (e == hipSuccess) e = dalloc(&c->mua_frac, TAM_MAX_MATERIALS + 1);
@@ -770,9 +785,23 @@ int tam_heat_run(tam_ctx* c, int32_t loops, tam_pulse* pulse)
     const Heat64Args a64 = heat64_args(c);
     const int boil = c->p.boil != 0, robin = c->p.robin_top != 0, dmg = c->p.damage_every_step != 0;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    const bool two = two_step(c);
     for (int p = 0; p < loops; ++p) {
         const double lon = (pulse && pulse->use_pulse) ? pulse->laserOn : 1.0;
         double t_after = (pulse ? pulse->time : c->time) + c->p.delt;
+        if (two && p + 1 < loops) {                  // steps p and p+1 in one launch
+            if (pulse) tam_pulse_advance(pulse, c->p.delt);
+            else c->time += c->p.delt;
+            const double lon1 = (pulse && pulse->use_pulse) ? pulse->laserOn : 1.0;
+            const double t_after1 = (pulse ? pulse->time : c->time) + c->p.delt;
+            HIPCHK(c, launch_heat_step2(a, c->cur, c->nxt, (float)lon, (float)lon1, t_after, t_after1, robin, dmg,
+                                        1, c->tile, c->kz, c->stream));
+            std::swap(c->cur, c->nxt);
+            if (pulse) tam_pulse_advance(pulse, c->p.delt);
+            else c->time += c->p.delt;
+            ++p;
+            continue;
+        }
         if (c->fp64) {
             HIPCHK(c, launch_heat64_step(a64, c->cur64, c->nxt64, lon, t_after, boil, robin, dmg, 1, c->stream));
             std::swap(c->cur64, c->nxt64);
@@ -1223,21 +1252,29 @@ int tam_tune_heat(tam_ctx* c, int32_t reps, int32_t* tile, int32_t* kz, double* 
         a.Q = qs;
         const int boil = c->p.boil != 0, robin = c->p.robin_top != 0, dmg = c->p.damage_every_step != 0;
         const int kzs[] = {4, 8, 12, 16, 32, 64};
-        for (int t = 0; t < heat_num_tiles() && e == hipSuccess; ++t)
+        const bool two_ok = !boil && c->lut;
+        for (int t = 0; t < heat_num_tiles() && e == hipSuccess; ++t) {
+            const bool two = heat_tile_steps(t) == 2;
+            if (two && !two_ok) continue;             // would run its one-step twin, timed already
             for (int kk : kzs) {
                 if (kk > c->p.nz && kk != 4) continue;
-                e = launch_heat_step(a, c->cur, c->nxt, 1.f, 0.0, boil, robin, dmg, 1, c->lut ? 1 : 0, t, kk,
-                                     c->stream);
+                // a two-step candidate launches two steps, timed per step
+                auto run = [&]() {
+                    return two ? launch_heat_step2(a, c->cur, c->nxt, 1.f, 1.f, 0.0, 0.0, robin, dmg, 1, t, kk,
+                                                   c->stream)
+                               : launch_heat_step(a, c->cur, c->nxt, 1.f, 0.0, boil, robin, dmg, 1, c->lut ? 1 : 0,
+                                                  t, kk, c->stream);
+                };
+                e = run();
                 if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
-                for (int r = 0; r < reps && e == hipSuccess; ++r)
-                    e = launch_heat_step(a, c->cur, c->nxt, 1.f, 0.0, boil, robin, dmg, 1, c->lut ? 1 : 0, t, kk,
-                                         c->stream);
+                for (int r = 0; r < reps && e == hipSuccess; ++r) e = run();
                 if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
                 if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
                 if (e != hipSuccess) break;
-                const double ms = elapsed(c) / reps;
+                const double ms = elapsed(c) / (reps * (two ? 2 : 1));
                 if (ms < best) { best = ms; bt = t; bk = kk; }
             }
+        }
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         (void)hipFree(om);
         (void)hipFree(th);

@@ -479,6 +479,234 @@ __global__ __launch_bounds__(TX* TY) void stencil_direct_kernel(const HeatArgs a
     direct_body<TX, TY, LUT>(a, Tin, Tout, lon, t_after, kz, flags, xcd_tile(), (int)gridDim.x, (int)gridDim.y);
 }
 
+// One step's damage at interior voxel v, dT = tn after the step
+// (3dFD.f90:450-461, the same arithmetic as the one-step kernels).  Omega and
+// the threshold mask are loaded on first use (from the prefetch when pom)
+// and kept in registers across the two steps of a launch; `have` says that
+// Omega changed and must be stored.
+__device__ __forceinline__ void damage_step(const HeatArgs& a, long v, long nv, float tn, double t_after, bool pom,
+                                            double om_pf, unsigned tm_pf, double& om, bool& have, unsigned& m,
+                                            bool& have_m)
+{
+    if (tn >= a.dT_lo_test && tn < a.dT_hi_test) {
+        const double T = a.t_ref + (double)tn;
+        if (T >= a.arr_Tlo && T < a.arr_Thi) {
+            const float arg = (float)(a.lnA - a.arr_dE / (a.arr_R * T));
+            if (!have) {
+                om = pom ? om_pf : a.omega[v];
+                have = true;
+            }
+            om = om + a.delt * (double)expf(arg);
+            if (om >= a.thr_min) {
+                if (!have_m) {
+                    m = pom ? tm_pf : a.tmask[v];
+                    have_m = true;
+                }
+                m = thres_chain(a.thres, a.tmask, v, nv, om, a.thr0, a.thr1, a.thr2, t_after, m);
+            }
+        }
+    }
+}
+
+// Two steps per launch (temporal blocking), class mode without boiling.
+//
+// A TX x TY block covers a footprint of TX x TY columns and owns the inner
+// (TX-2) x (TY-2) of them.  Marching up z, it computes the first step on
+// every footprint column (x-y neighbours loaded directly, as in direct_body),
+// hands each first-step plane to the second step through a double-buffered
+// LDS plane (one barrier per plane), and computes the second step one plane
+// behind on the owned columns.  The intermediate field never reaches HBM:
+// per two steps a voxel's dT is read and written once, its class and source
+// read once, Omega read and written at most once.  The footprint ring
+// repeats first-step work: TX*TY / ((TX-2)*(TY-2)) in x-y, (kz+2)/kz in z.
+//
+// Both steps use the one-step kernels' face coefficients (they only change in
+// the rebuild) and arithmetic, in the same order, so the field, Omega and
+// threshold times are bit-identical to two one-step launches.  A voxel's
+// damage, negative-temperature flag and boundary energy are applied by its
+// owning lane only, step 1 before step 2.
+template <int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void stencil_t2_kernel(const HeatArgs a, const float* __restrict__ Tin,
+                                                            float* __restrict__ Tout, float lon0, float lon1,
+                                                            double ta0, double ta1, int kz, unsigned flags)
+{
+    if (a.dyn_lon) {              // graph replay: both steps' values from device memory
+        lon0 = a.dyn_lon[0];
+        lon1 = a.dyn_lon[1];
+        ta0 = a.dyn_tafter[0];
+        ta1 = a.dyn_tafter[1];
+    }
+    constexpr int OX = TX - 2, OY = TY - 2;
+    __shared__ float sLut[3 * 256 + 16];
+    __shared__ float2 sRob[16];
+    __shared__ float sT1[2][TY][TX];            // first-step plane, buffer = plane parity
+    const uint8_t* __restrict__ cla = a.cls;
+    const float* __restrict__ srca = a.src;
+    const long px = a.px, pxy = a.pxy;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const TileId tl = xcd_tile();
+    const int i = tl.x * OX + tx, j = tl.y * OY + ty;            // footprint column; 0 and n+1 are halo
+    const bool real = i >= 1 && i <= nx && j >= 1 && j <= ny;    // the first step updates this column
+    const bool own = real && tx >= 1 && tx <= TX - 2 && ty >= 1 && ty <= TY - 2;
+    const bool bnd_ij = i == 1 || i == nx || j == 1 || j == ny;
+    // dT and class of the lane's own column (a halo column for lanes outside the grid)
+    const long ch = (long)min(i, nx + 1) + px * min(j, ny + 1);
+    // x-y neighbours and source: the nearest interior column, so every address
+    // is valid; lanes that are not `real` discard what they compute from them
+    const long ci = (long)min(max(i, 1), nx) + px * min(max(j, 1), ny);
+    const int k0 = tl.z * kz + 1;
+    const int k1 = min(k0 + kz, nz + 1);
+    const bool robin = flags & F_ROBIN, damage = flags & F_DAMAGE;
+    const bool energy = (flags & F_ENERGY) && (tl.x == 0 || tl.x == (int)gridDim.x - 1 || tl.y == 0 ||
+                                               tl.y == (int)gridDim.y - 1 || k0 == 1 || k1 == nz + 1);
+    for (int t = threadIdx.x; t < 3 * 256 + 16; t += TX * TY) sLut[t] = a.glut[t];
+    if (threadIdx.x < 16) sRob[threadIdx.x] = a.robin_lut[threadIdx.x];
+    __syncthreads();
+
+    const long nxy = (long)nx * ny, nv = nxy * nz;
+    const long vcol = (long)(i - 1) + (long)nx * (j - 1);        // interior index of the column (own lanes)
+    const float wlo = a.dT_lo_test - a.dT_win_pad, whi = a.dT_hi_test + a.dT_win_pad;
+    // first step at plane p: dT0 at p-1, p (z0m, z0c), classes at p-1, p
+    int p = k0 - 1;
+    float z0m = p >= 1 ? Tin[ch + pxy * (p - 1)] : 0.f;
+    float z0c = Tin[ch + pxy * p];
+    int cm = p >= 1 ? cla[ch + pxy * (p - 1)] & TAM_CLASS_MASK : 0;
+    unsigned ccr = cla[ch + pxy * p];
+    // second step at plane k = p-1: dT1 at k-1, k (u1m, u1c) and the first
+    // step's face coefficients, source and class at k
+    float u1m = 0.f, u1c = 0.f;
+    float Gxm = 0.f, Gxp = 0.f, Gym = 0.f, Gyp = 0.f, Gzm = 0.f, Gzp = 0.f, Sk = 0.f;
+    int Ck = 0;
+    float e_acc = 0.f;
+    for (; p <= k1; ++p) {
+        const bool interior = p >= 1 && p <= nz;              // else a halo plane: dT fixed
+        const int k = p - 1;
+        const bool step2 = p > k0;                            // second step at k in this iteration
+        // ---- loads of plane p (and the column at p+1); Omega of (i, j, k)
+        float z0p = 0.f, txm = 0.f, txp = 0.f, tym = 0.f, typ = 0.f, s_p = 0.f;
+        unsigned cpr = 0;
+        if (p <= nz) {                                        // plane p+1 <= nz+1 exists
+            z0p = Tin[ch + pxy * (p + 1)];
+            cpr = cla[ch + pxy * (p + 1)];
+        }
+        if (interior) {
+            const long c = ci + pxy * p;
+            txm = Tin[c - 1];
+            txp = Tin[c + 1];
+            tym = Tin[c - px];
+            typ = Tin[c + px];
+            s_p = srca[c];
+        }
+        const long v = vcol + nxy * (k - 1);
+        const bool pom = step2 && damage && own && u1c >= wlo && u1c < whi;
+        double om_pf = 0.0;
+        unsigned tm_pf = 0;
+        if (pom) {
+            om_pf = a.omega[v];
+            tm_pf = a.tmask[v];
+        }
+        // ---- first step at plane p
+        float t1p = z0c;                                      // halo planes and columns keep dT
+        float gxm = 0.f, gxp = 0.f, gym = 0.f, gyp = 0.f, gzm = 0.f, gzp = 0.f;
+        const int cc = ccr & TAM_CLASS_MASK;
+        if (interior) {
+            const int cp = cpr & TAM_CLASS_MASK;
+            if (__all((ccr & TAM_CLASS_HOM) != 0)) {
+                gxm = gxp = sLut[cc * 17];
+                gym = gyp = sLut[256 + cc * 17];
+                gzm = gzp = sLut[512 + cc * 17];
+            } else {
+                const long c = ci + pxy * p;
+                const int cxm = cla[c - 1] & TAM_CLASS_MASK, cxp = cla[c + 1] & TAM_CLASS_MASK;
+                const int cym = cla[c - px] & TAM_CLASS_MASK, cyp = cla[c + px] & TAM_CLASS_MASK;
+                gxm = sLut[cxm * 16 + cc];
+                gxp = sLut[cc * 16 + cxp];
+                gym = sLut[256 + cym * 16 + cc];
+                gyp = sLut[256 + cc * 16 + cyp];
+                gzp = sLut[512 + cc * 16 + cp];
+                gzm = sLut[512 + cm * 16 + cc];
+            }
+            float tp = z0p;
+            if (robin && p == nz) {
+                const float2 rb = sRob[cc];
+                tp = fmaf(rb.x, z0c, rb.y);
+            }
+            const float inc = face_sum(z0c, gxm, txm, gxp, txp, gym, tym, gyp, typ, gzm, z0m, gzp, tp);
+            if (real) t1p = step_update(z0c, inc, lon0, s_p);
+            if (own && p >= k0 && p < k1) {
+                if (z0c < a.dT_neg) a.flags[0] = 1;             // 3dFD.f90:179-182
+                if (energy && (bnd_ij || p == 1 || p == nz)) {
+                    float e = 0.f;
+                    if (i == 1) e += gxm * (txm - z0c);
+                    if (i == nx) e += gxp * (txp - z0c);
+                    if (j == 1) e += gym * (tym - z0c);
+                    if (j == ny) e += gyp * (typ - z0c);
+                    if (p == 1) e += gzm * (z0m - z0c);
+                    if (p == nz) e += gzp * (tp - z0c);
+                    e_acc -= sLut[768 + cc] * e;
+                }
+            }
+        }
+        // ---- second step at plane k = p-1 (dT1 of plane k is in sT1[k & 1])
+        if (step2 && own) {
+            const float xm = sT1[k & 1][ty][tx - 1], xp = sT1[k & 1][ty][tx + 1];
+            const float ym = sT1[k & 1][ty - 1][tx], yp = sT1[k & 1][ty + 1][tx];
+            float tp = t1p;
+            if (robin && k == nz) {
+                const float2 rb = sRob[Ck];
+                tp = fmaf(rb.x, u1c, rb.y);
+            }
+            const float inc = face_sum(u1c, Gxm, xm, Gxp, xp, Gym, ym, Gyp, yp, Gzm, u1m, Gzp, tp);
+            const float tn = step_update(u1c, inc, lon1, Sk);
+            if (u1c < a.dT_neg) a.flags[0] = 1;
+            Tout[ch + pxy * k] = tn;
+            if (damage) {                                       // step 1, then step 2
+                double om = 0.0;
+                unsigned m = 0;
+                bool have = false, have_m = false;
+                damage_step(a, v, nv, u1c, ta0, pom, om_pf, tm_pf, om, have, m, have_m);
+                damage_step(a, v, nv, tn, ta1, pom, om_pf, tm_pf, om, have, m, have_m);
+                if (have) a.omega[v] = om;
+            }
+            if (energy && (bnd_ij || k == 1 || k == nz)) {
+                float e = 0.f;
+                if (i == 1) e += Gxm * (xm - u1c);
+                if (i == nx) e += Gxp * (xp - u1c);
+                if (j == 1) e += Gym * (ym - u1c);
+                if (j == ny) e += Gyp * (yp - u1c);
+                if (k == 1) e += Gzm * (u1m - u1c);
+                if (k == nz) e += Gzp * (tp - u1c);
+                e_acc -= sLut[768 + Ck] * e;
+            }
+        }
+        // plane p's buffer last held plane p-2, whose readers passed the
+        // previous iteration's barrier
+        sT1[p & 1][ty][tx] = t1p;
+        __syncthreads();
+        u1m = u1c;
+        u1c = t1p;
+        z0m = z0c;
+        z0c = z0p;
+        cm = cc;
+        ccr = cpr;
+        Gxm = gxm; Gxp = gxp; Gym = gym; Gyp = gyp; Gzm = gzm; Gzp = gzp;
+        Sk = s_p;
+        Ck = cc;
+    }
+    if (energy) {
+        __shared__ double sE[TX * TY / 64];
+        const double w = wave_sum_d((double)e_acc);
+        if ((threadIdx.x & 63) == 0) sE[threadIdx.x >> 6] = w;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int q = 0; q < TX * TY / 64; ++q) t += sE[q];
+            a.eblk[tl.id] += t;
+        }
+    }
+}
+
 // Many grids in one launch: block b works on the instance whose tile range
 // [tile_start[i], tile_start[i + 1]) holds b, at that instance's local tile
 // (x fastest).  Every instance uses the same tile shape and step flags.
@@ -548,6 +776,16 @@ hipError_t launch_t(const HeatArgs& a, const float* Tin, float* Tout, float lon,
     return hipGetLastError();
 }
 
+template <int TX, int TY>
+hipError_t launch_t2(const HeatArgs& a, const float* Tin, float* Tout, float lon0, float lon1, double ta0,
+                     double ta1, int kz, unsigned flags, hipStream_t s)
+{
+    const dim3 grid((a.nx + TX - 3) / (TX - 2), (a.ny + TY - 3) / (TY - 2), (a.nz + kz - 1) / kz);
+    hipLaunchKernelGGL((stencil_t2_kernel<TX, TY>), grid, dim3(TX * TY), 0, s, a, Tin, Tout, lon0, lon1, ta0, ta1,
+                       kz, flags);
+    return hipGetLastError();
+}
+
 template <int TX, int TY, bool LUT>
 hipError_t launch_batch_t(const HeatBatchEntry* tab, int n, const int* tile_start, int total_tiles, float lon,
                           double t, int kz, unsigned flags, int parity, hipStream_t s)
@@ -585,12 +823,40 @@ hipError_t launch_heat_batch(const HeatBatchEntry* tab, int n, const int* tile_s
     return hipErrorInvalidValue;      // batches use the direct shapes 3-5
 }
 
-// launch shapes: 0-2 stage x-y planes in LDS, 3-5 load neighbours directly
-int heat_num_tiles() { return 6; }
+// launch shapes: 0-2 stage x-y planes in LDS, 3-5 load neighbours directly,
+// 6-8 run two steps per launch (stencil_t2_kernel; a single step, or a step
+// outside class mode or with boiling, uses the direct shape named beside it)
+int heat_num_tiles() { return 9; }
 const char* heat_tile_name(int t)
 {
-    static const char* names[] = {"lds-64x4", "lds-32x8", "lds-64x8", "direct-64x4", "direct-32x8", "direct-64x8"};
-    return (t >= 0 && t < 6) ? names[t] : "?";
+    static const char* names[] = {"lds-64x4",    "lds-32x8",     "lds-64x8",    "direct-64x4", "direct-32x8",
+                                  "direct-64x8", "two-step-32x8", "two-step-32x16", "two-step-64x8"};
+    return (t >= 0 && t < 9) ? names[t] : "?";
+}
+int heat_tile_steps(int tile) { return tile >= 6 && tile <= 8 ? 2 : 1; }
+
+long heat_max_blocks(int nx, int ny, int nz)
+{
+    // one-step shapes: tiles of at least 32 x 4; two-step shapes: at least 30 x 6 owned columns
+    const long one = (long)((nx + 31) / 32) * ((ny + 3) / 4) * nz;
+    const long two = (long)((nx + 29) / 30) * ((ny + 5) / 6) * nz;
+    return one > two ? one : two;
+}
+
+hipError_t launch_heat_step2(const HeatArgs& a, const float* Tin, float* Tout, float lon0, float lon1,
+                             double time_after0, double time_after1, int robin, int damage, int energy, int tile,
+                             int kz, hipStream_t s)
+{
+    const unsigned f = (robin ? F_ROBIN : 0u) | (damage ? F_DAMAGE : 0u) | (energy ? F_ENERGY : 0u);
+    if (kz < 1) kz = 1;
+    if (kz > a.nz) kz = a.nz;
+    if (!a.cls || !a.glut) return hipErrorInvalidValue;      // class mode only
+    switch (tile) {
+    case 6: return launch_t2<32, 8>(a, Tin, Tout, lon0, lon1, time_after0, time_after1, kz, f, s);
+    case 7: return launch_t2<32, 16>(a, Tin, Tout, lon0, lon1, time_after0, time_after1, kz, f, s);
+    case 8: return launch_t2<64, 8>(a, Tin, Tout, lon0, lon1, time_after0, time_after1, kz, f, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_heat_step(const HeatArgs& a, const float* Tin, float* Tout, float lon, double time_after,
@@ -601,6 +867,8 @@ hipError_t launch_heat_step(const HeatArgs& a, const float* Tin, float* Tout, fl
                        (energy ? F_ENERGY : 0u);
     if (kz < 1) kz = 1;
     if (kz > a.nz) kz = a.nz;
+    if (tile == 6 || tile == 7) tile = 4;       // two-step shapes: one step on direct-32x8 / direct-64x8
+    else if (tile == 8) tile = 5;
     switch (tile * 2 + (lut ? 1 : 0)) {
     case 0: return launch_t<64, 4, false, false>(a, Tin, Tout, lon, time_after, kz, f, s);
     case 1: return launch_t<64, 4, true, false>(a, Tin, Tout, lon, time_after, kz, f, s);

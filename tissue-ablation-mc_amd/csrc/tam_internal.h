@@ -28,8 +28,9 @@
 // "ThresTime(k) == 0" read from bit k of the voxel's mask m instead of from
 // the FP64 time, so a step reads one byte rather than up to three doubles.
 // A time written as 0.0 leaves its bit clear, as ThresTime(k) stays 0 then.
-__device__ __forceinline__ void thres_chain(double* thres, uint8_t* tmask, long v, long nv, double om, double thr0,
-                                            double thr1, double thr2, double t, unsigned m)
+// Returns the voxel's mask after the call.
+__device__ __forceinline__ unsigned thres_chain(double* thres, uint8_t* tmask, long v, long nv, double om,
+                                                double thr0, double thr1, double thr2, double t, unsigned m)
 {
     int k = -1;
     if (!(m & 1u) && om >= thr0) k = 0;
@@ -37,8 +38,12 @@ __device__ __forceinline__ void thres_chain(double* thres, uint8_t* tmask, long 
     else if (!(m & 4u) && om >= thr2) k = 2;
     if (k >= 0) {
         thres[v + k * nv] = t;
-        if (t != 0.0) tmask[v] = (uint8_t)(m | (1u << k));
+        if (t != 0.0) {
+            m |= 1u << k;
+            tmask[v] = (uint8_t)m;
+        }
     }
+    return m;
 }
 
 struct MatEntry {          // per material, in device memory
@@ -272,6 +277,16 @@ hipError_t launch_heat_step(const HeatArgs& a, const float* Tin, float* Tout, fl
                             hipStream_t s);
 int heat_num_tiles();
 int heat_tile_dims(int tile, int* tx, int* ty);
+// 2 for the two-step shapes (launch_heat_step2), 1 otherwise
+int heat_tile_steps(int tile);
+// two steps Tin -> Tout in one launch (two-step shapes; class mode without
+// boiling); step 1 uses (lon0, time_after0), step 2 (lon1, time_after1), or
+// a.dyn_lon[0..1] / a.dyn_tafter[0..1] when a.dyn_lon is set
+hipError_t launch_heat_step2(const HeatArgs& a, const float* Tin, float* Tout, float lon0, float lon1,
+                             double time_after0, double time_after1, int robin, int damage, int energy, int tile,
+                             int kz, hipStream_t s);
+// upper bound of the block count of any heat launch shape (energy slots)
+long heat_max_blocks(int nx, int ny, int nz);
 // one step of every instance of a table (direct shapes 3-5 only); parity 0
 // reads cur and writes nxt, parity 1 the reverse
 hipError_t launch_heat_batch(const HeatBatchEntry* tab, int n, const int* tile_start, int total_tiles, float lon,

@@ -11,6 +11,9 @@ that state.  This script splits the two:
     # the byte-counter calibration) with a given launch shape
     python tissue-ablation-mc_amd/stencil_state.py run /tmp/c1_state.npz --tile 4 --kz 8
 
+    # time several launch shapes (tile:kz), each from the saved state
+    python tissue-ablation-mc_amd/stencil_state.py run /tmp/c1_state.npz --shapes 4:8,8:8,8:16
+
 The saved fields are the temperature (with halo), Omega, the threshold times,
 water, Q, jmean (the next cycle's source) and the rebuilt kappa, density,
 heat capacity and rhokap (0 in removed voxels).
@@ -43,6 +46,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--tile", type=int, default=4)
     ap.add_argument("--kz", type=int, default=8)
+    ap.add_argument("--shapes", default=None, help="comma list of tile:kz (overrides --tile/--kz)")
     args = ap.parse_args()
     n = args.grid
     prob = configs.config1(n=(n, n, n), packets=int(1e8), steps=10000, cycles=args.cycles)
@@ -58,21 +62,25 @@ def main():
         g.close()
         return
     d = np.load(args.path)
-    g = lib.TissueGrid(prob.params)
-    g.set_optics(prob.mat, prob.table, d["rhokap"])          # removed voxels: rhokap 0
-    g.set_thermal(d["temp"], d["kappa"], d["density"], d["heatcap"])
-    for k in ("omega", "thres", "water", "q", "jmean"):
-        g.upload(FIELDS[k], d[k])
-    log("loaded", args.path)
-    log("copy GB/s", g.copy_bandwidth(256 << 20, reps=3))
-    g.set_heat_launch(args.tile, args.kz)
-    done = 0
-    while done < args.steps:
-        k = min(50, args.steps - done)
-        g.heat_run(k)
-        done += k
-        log(f"{k} steps: {g.timing().heat_ms / k * 1e3:.2f} us/step")
-    g.close()
+    shapes = ([tuple(int(x) for x in s.split(":")) for s in args.shapes.split(",")] if args.shapes
+              else [(args.tile, args.kz)])
+    for n_shape, (tile, kz) in enumerate(shapes):
+        g = lib.TissueGrid(prob.params)
+        g.set_optics(prob.mat, prob.table, d["rhokap"])          # removed voxels: rhokap 0
+        g.set_thermal(d["temp"], d["kappa"], d["density"], d["heatcap"])
+        for k in ("omega", "thres", "water", "q", "jmean"):
+            g.upload(FIELDS[k], d[k])
+        log("loaded", args.path)
+        if n_shape == 0:
+            log("copy GB/s", g.copy_bandwidth(256 << 20, reps=3))
+        g.set_heat_launch(tile, kz)
+        done = 0
+        while done < args.steps:
+            k = min(50, args.steps - done)
+            g.heat_run(k)
+            done += k
+            log(f"tile {tile} kz {kz}, {k} steps: {g.timing().heat_ms / k * 1e3:.2f} us/step")
+        g.close()
 
 
 if __name__ == "__main__":

@@ -22,8 +22,9 @@ class CycleRunner:
         self.time = 0.0
         self.cycle = 0
         self.totals = dict(packets=0.0, crossings=0.0, interactions=0.0, tally_adds=0.0, mc_ms=0.0, heat_ms=0.0,
-                           heat_launches=0, rebuild_ms=0.0, normalise_ms=0.0, arrhenius_ms=0.0,
+                           heat_steps=0, rebuild_ms=0.0, normalise_ms=0.0, arrhenius_ms=0.0,
                            voxel_updates=0.0)
+        self.heat_by_shape = {}      # (tile, kz) -> [heat ms, steps] of the stepwise cycles
 
     def reset(self):
         """Restore the problem's initial tissue (optics, temperature, damage)."""
@@ -36,6 +37,7 @@ class CycleRunner:
         self.cycle = 0
         for k in self.totals:
             self.totals[k] = 0.0
+        self.heat_by_shape = {}
 
     def run_cycle(self):
         if self.graph:
@@ -60,7 +62,10 @@ class CycleRunner:
         T["tally_adds"] += st.tally_adds
         T["mc_ms"] += st.kernel_ms
         T["heat_ms"] += t.heat_ms
-        T["heat_launches"] += t.heat_steps
+        T["heat_steps"] += t.heat_steps
+        acc = self.heat_by_shape.setdefault((int(t.tile), int(t.kz)), [0.0, 0])
+        acc[0] += t.heat_ms
+        acc[1] += t.heat_steps
         T["rebuild_ms"] += t.rebuild_ms
         T["normalise_ms"] += t.normalise_ms
         T["arrhenius_ms"] += 0.0 if p.params.damage_every_step else t.arrhenius_ms
