@@ -124,7 +124,8 @@ typedef struct {
 typedef struct {
     double mc_ms, normalise_ms, heat_ms, arrhenius_ms, rebuild_ms;   /* last call of each */
     double heat_steps;           /* steps in the last tam_heat_run */
-    double heat_step_ms;         /* mean device time of one stencil launch in the last tam_heat_run */
+    double heat_step_ms;         /* mean device time per step in the last tam_heat_run (a two-step
+                                    launch counts as two steps) */
     double class_mode;           /* 1: stencil runs on thermal classes + LDS coefficient table */
     double tile, kz;             /* stencil launch shape in use */
     double cycle_ms;             /* device time of the last timed tam_cycle (transfer + graph) */
@@ -273,11 +274,13 @@ int tam_synchronize(tam_ctx* ctx);
  *   i: npackets[i] packets on Philox stream stream0 + i of key seed, as
  *   tam_mc_run would give); stats[i] per instance when stats is not NULL.
  * tam_batch_heat_run: `loops` steps of every instance, one launch per step
- *   (laser on; the source from each instance's last tam_normalise).
+ *   or, with a two-step shape in class mode, per pair of steps (laser on;
+ *   the source from each instance's last tam_normalise).
  * tam_batch_tune: times transport shapes (blocks per CU x random-phase
  *   batch) on sample_packets[i] per instance and stencil shapes (direct
- *   tiles 3-5 x z depths) over heat_reps steps without advancing the state,
- *   and keeps the fastest. */
+ *   tiles 3-5 and two-step tiles 6-8 x z depths) over heat_reps steps
+ *   without advancing the state, and keeps the fastest.
+ * tam_batch_set_launch: tile 3..8 as tam_set_heat_launch. */
 typedef struct tam_batch tam_batch;
 int  tam_batch_create(tam_ctx* const* ctxs, int32_t n, tam_batch** out);
 void tam_batch_destroy(tam_batch* b);

@@ -1396,23 +1396,40 @@ int batch_heat(tam_batch* b, int steps, int tile, int kz, bool advance)
         tab[(size_t)i].cur = c->cur;
         tab[(size_t)i].nxt = c->nxt;
     }
-    std::vector<int> start;
-    const int total = batch_tiles(b, tile, kz, start);
-    BCHK(b, hipMemcpyAsync(b->dheat, tab.data(), sizeof(HeatBatchEntry) * (size_t)n, hipMemcpyHostToDevice,
-                           b->stream));
-    BCHK(b, hipMemcpyAsync(b->dtile, start.data(), sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice, b->stream));
     const tam_params& p0 = b->ctx[0]->p;
     const int robin = p0.robin_top != 0, lut = b->ctx[0]->lut ? 1 : 0;
+    // two-step shapes pair the steps (class mode); single steps run on the
+    // one-step twin shape, direct 32x8 for 6 and 7, direct 64x8 for 8
+    const bool two = heat_tile_steps(tile) == 2 && lut;
+    const int one = tile == 8 ? 5 : (tile >= 6 ? 4 : tile);
+    std::vector<int> start(2 * ((size_t)n + 1), 0), s1, s2;
+    const int total1 = batch_tiles(b, one, kz, s1);
+    const int total2 = two ? batch_tiles(b, tile, kz, s2) : 0;
+    std::copy(s1.begin(), s1.end(), start.begin());
+    if (two) std::copy(s2.begin(), s2.end(), start.begin() + (n + 1));
+    BCHK(b, hipMemcpyAsync(b->dheat, tab.data(), sizeof(HeatBatchEntry) * (size_t)n, hipMemcpyHostToDevice,
+                           b->stream));
+    BCHK(b, hipMemcpyAsync(b->dtile, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice, b->stream));
     BCHK(b, hipEventRecord(b->ev0, b->stream));
-    for (int k = 0; k < steps; ++k)
-        BCHK(b, launch_heat_batch(b->dheat, n, b->dtile, total, 1.f, 0.0, 0, robin, 0, advance ? 1 : 0, lut, tile,
-                                  kz, advance ? (k & 1) : 0, b->stream));
+    int par = 0;                       // launches so far, mod 2: which of cur / nxt holds the state
+    for (int k = 0; k < steps; ++par) {
+        const int pz = advance ? (par & 1) : 0;
+        if (two && k + 1 < steps) {
+            BCHK(b, launch_heat_batch2(b->dheat, n, b->dtile + (n + 1), total2, 1.f, 1.f, 0.0, 0.0, robin, 0,
+                                       advance ? 1 : 0, tile, kz, pz, b->stream));
+            k += 2;
+        } else {
+            BCHK(b, launch_heat_batch(b->dheat, n, b->dtile, total1, 1.f, 0.0, 0, robin, 0, advance ? 1 : 0, lut, one,
+                                      kz, pz, b->stream));
+            k += 1;
+        }
+    }
     BCHK(b, hipEventRecord(b->ev1, b->stream));
     BCHK(b, hipStreamSynchronize(b->stream));
     b->last_ms = bms(b);
     if (advance) {
         for (tam_ctx* c : b->ctx) {
-            if (steps & 1) std::swap(c->cur, c->nxt);
+            if (par & 1) std::swap(c->cur, c->nxt);
             c->time += steps * c->p.delt;
             c->timing.heat_steps = steps;
         }
@@ -1453,7 +1470,8 @@ int tam_batch_create(tam_ctx* const* ctxs, int32_t n, tam_batch** out)
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     if (e == hipSuccess) e = hipMalloc((void**)&b->dmc, sizeof(McArgs) * (size_t)n);
     if (e == hipSuccess) e = hipMalloc((void**)&b->dheat, sizeof(HeatBatchEntry) * (size_t)n);
-    if (e == hipSuccess) e = hipMalloc((void**)&b->dtile, sizeof(int) * ((size_t)n + 1));
+    // two tile tables: the launch shape's and its one-step twin's
+    if (e == hipSuccess) e = hipMalloc((void**)&b->dtile, sizeof(int) * 2 * ((size_t)n + 1));
     if (e != hipSuccess) {
         g_batch_err = std::string("batch allocation: ") + hipGetErrorString(e);
         tam_batch_destroy(b);
@@ -1527,8 +1545,8 @@ int tam_batch_heat_run(tam_batch* b, int32_t loops)
 int tam_batch_set_launch(tam_batch* b, int32_t tile, int32_t kz, int32_t blocks_per_cu, int32_t rng_batch)
 {
     if (!b) return TAM_ERR_ARG;
-    if (tile < 3 || tile > 5 || kz < 1 || blocks_per_cu < 1 || blocks_per_cu > 64 || rng_batch < 1 || rng_batch > 64)
-        return bfail(b, TAM_ERR_ARG, "batch launch: tile 3..5, kz >= 1, blocks_per_cu 1..64, rng_batch 1..64");
+    if (tile < 3 || tile > 8 || kz < 1 || blocks_per_cu < 1 || blocks_per_cu > 64 || rng_batch < 1 || rng_batch > 64)
+        return bfail(b, TAM_ERR_ARG, "batch launch: tile 3..8, kz >= 1, blocks_per_cu 1..64, rng_batch 1..64");
     b->tile = tile;
     b->kz = kz;
     b->bpc = blocks_per_cu;
@@ -1558,10 +1576,11 @@ int tam_batch_tune(tam_batch* b, const int64_t* sample_packets, int32_t heat_rep
     b->bpc = bb;
     b->rng_batch = bt;
     for (tam_ctx* c : b->ctx) BCHK(b, launch_fill_f64(c->tally, c->nvox, 0.0, b->stream));
-    // stencil: direct shapes x z depths, state not advanced
+    // stencil: direct and two-step shapes x z depths, state not advanced
+    // (heat_reps steps per candidate: two-step shapes run them in pairs)
     best = 1e300;
     int btl = b->tile, bk = b->kz;
-    for (int t = 3; t <= 5; ++t)
+    for (int t = 3; t <= 8; ++t)
         for (int k : {4, 8, 16, 32}) {
             rc = batch_heat(b, 1, t, k, false);            // warm-up
             if (rc == TAM_OK) rc = batch_heat(b, heat_reps, t, k, false);

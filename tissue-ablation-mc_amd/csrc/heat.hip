@@ -525,17 +525,13 @@ __device__ __forceinline__ void damage_step(const HeatArgs& a, long v, long nv, 
 // threshold times are bit-identical to two one-step launches.  A voxel's
 // damage, negative-temperature flag and boundary energy are applied by its
 // owning lane only, step 1 before step 2.
+// tl: this block's tile (owned (TX-2) x (TY-2) columns); gx, gy: tiles per
+// row and column of the grid.
 template <int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void stencil_t2_kernel(const HeatArgs a, const float* __restrict__ Tin,
-                                                            float* __restrict__ Tout, float lon0, float lon1,
-                                                            double ta0, double ta1, int kz, unsigned flags)
+__device__ __forceinline__ void t2_body(const HeatArgs& a, const float* __restrict__ Tin, float* __restrict__ Tout,
+                                        float lon0, float lon1, double ta0, double ta1, int kz, unsigned flags,
+                                        const TileId tl, int gx, int gy)
 {
-    if (a.dyn_lon) {              // graph replay: both steps' values from device memory
-        lon0 = a.dyn_lon[0];
-        lon1 = a.dyn_lon[1];
-        ta0 = a.dyn_tafter[0];
-        ta1 = a.dyn_tafter[1];
-    }
     constexpr int OX = TX - 2, OY = TY - 2;
     __shared__ float sLut[3 * 256 + 16];
     __shared__ float2 sRob[16];
@@ -545,7 +541,6 @@ __global__ __launch_bounds__(TX* TY) void stencil_t2_kernel(const HeatArgs a, co
     const long px = a.px, pxy = a.pxy;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const TileId tl = xcd_tile();
     const int i = tl.x * OX + tx, j = tl.y * OY + ty;            // footprint column; 0 and n+1 are halo
     const bool real = i >= 1 && i <= nx && j >= 1 && j <= ny;    // the first step updates this column
     const bool own = real && tx >= 1 && tx <= TX - 2 && ty >= 1 && ty <= TY - 2;
@@ -558,8 +553,8 @@ __global__ __launch_bounds__(TX* TY) void stencil_t2_kernel(const HeatArgs a, co
     const int k0 = tl.z * kz + 1;
     const int k1 = min(k0 + kz, nz + 1);
     const bool robin = flags & F_ROBIN, damage = flags & F_DAMAGE;
-    const bool energy = (flags & F_ENERGY) && (tl.x == 0 || tl.x == (int)gridDim.x - 1 || tl.y == 0 ||
-                                               tl.y == (int)gridDim.y - 1 || k0 == 1 || k1 == nz + 1);
+    const bool energy = (flags & F_ENERGY) &&
+                        (tl.x == 0 || tl.x == gx - 1 || tl.y == 0 || tl.y == gy - 1 || k0 == 1 || k1 == nz + 1);
     for (int t = threadIdx.x; t < 3 * 256 + 16; t += TX * TY) sLut[t] = a.glut[t];
     if (threadIdx.x < 16) sRob[threadIdx.x] = a.robin_lut[threadIdx.x];
     __syncthreads();
@@ -707,13 +702,28 @@ __global__ __launch_bounds__(TX* TY) void stencil_t2_kernel(const HeatArgs a, co
     }
 }
 
+template <int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void stencil_t2_kernel(const HeatArgs a, const float* __restrict__ Tin,
+                                                            float* __restrict__ Tout, float lon0, float lon1,
+                                                            double ta0, double ta1, int kz, unsigned flags)
+{
+    if (a.dyn_lon) {              // graph replay: both steps' values from device memory
+        lon0 = a.dyn_lon[0];
+        lon1 = a.dyn_lon[1];
+        ta0 = a.dyn_tafter[0];
+        ta1 = a.dyn_tafter[1];
+    }
+    t2_body<TX, TY>(a, Tin, Tout, lon0, lon1, ta0, ta1, kz, flags, xcd_tile(), (int)gridDim.x, (int)gridDim.y);
+}
+
 // Many grids in one launch: block b works on the instance whose tile range
 // [tile_start[i], tile_start[i + 1]) holds b, at that instance's local tile
 // (x fastest).  Every instance uses the same tile shape and step flags.
-template <int TX, int TY, bool LUT>
-__global__ __launch_bounds__(TX* TY) void stencil_batch_kernel(const HeatBatchEntry* __restrict__ tab, int n,
-                                                               const int* __restrict__ tile_start, float lon,
-                                                               double t_after, int kz, unsigned flags, int parity)
+// Returns the instance; tl, gx, gy get the local tile and the instance's
+// tiles per row and column for owned tiles of ox x oy columns.
+__device__ __forceinline__ int batch_tile(const HeatBatchEntry* __restrict__ tab, int n,
+                                          const int* __restrict__ tile_start, int ox, int oy, TileId& tl, int& gx,
+                                          int& gy)
 {
     const int b = blockIdx.x;
     int lo = 0, hi = n - 1;                       // last i with tile_start[i] <= b
@@ -723,16 +733,42 @@ __global__ __launch_bounds__(TX* TY) void stencil_batch_kernel(const HeatBatchEn
         else hi = mid - 1;
     }
     const HeatBatchEntry& e = tab[lo];
-    const int gx = (e.a.nx + TX - 1) / TX, gy = (e.a.ny + TY - 1) / TY;
+    gx = (e.a.nx + ox - 1) / ox;
+    gy = (e.a.ny + oy - 1) / oy;
     const int local = b - tile_start[lo];
-    TileId tl;
     tl.x = local % gx;
     tl.y = (local / gx) % gy;
     tl.z = local / (gx * gy);
     tl.id = local;
+    return lo;
+}
+
+template <int TX, int TY, bool LUT>
+__global__ __launch_bounds__(TX* TY) void stencil_batch_kernel(const HeatBatchEntry* __restrict__ tab, int n,
+                                                               const int* __restrict__ tile_start, float lon,
+                                                               double t_after, int kz, unsigned flags, int parity)
+{
+    TileId tl;
+    int gx, gy;
+    const HeatBatchEntry& e = tab[batch_tile(tab, n, tile_start, TX, TY, tl, gx, gy)];
     const float* Tin = parity ? e.nxt : e.cur;
     float* Tout = parity ? e.cur : e.nxt;
     direct_body<TX, TY, LUT>(e.a, Tin, Tout, lon, t_after, kz, flags, tl, gx, gy);
+}
+
+// Two steps of every instance in one launch (class mode, as stencil_t2_kernel).
+template <int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void stencil_t2_batch_kernel(const HeatBatchEntry* __restrict__ tab, int n,
+                                                                  const int* __restrict__ tile_start, float lon0,
+                                                                  float lon1, double ta0, double ta1, int kz,
+                                                                  unsigned flags, int parity)
+{
+    TileId tl;
+    int gx, gy;
+    const HeatBatchEntry& e = tab[batch_tile(tab, n, tile_start, TX - 2, TY - 2, tl, gx, gy)];
+    const float* Tin = parity ? e.nxt : e.cur;
+    float* Tout = parity ? e.cur : e.nxt;
+    t2_body<TX, TY>(e.a, Tin, Tout, lon0, lon1, ta0, ta1, kz, flags, tl, gx, gy);
 }
 
 // Arrhenius at the reference cadence: every interior voxel, the full
@@ -786,6 +822,15 @@ hipError_t launch_t2(const HeatArgs& a, const float* Tin, float* Tout, float lon
     return hipGetLastError();
 }
 
+template <int TX, int TY>
+hipError_t launch_t2_batch_t(const HeatBatchEntry* tab, int n, const int* tile_start, int total_tiles, float lon0,
+                             float lon1, double ta0, double ta1, int kz, unsigned flags, int parity, hipStream_t s)
+{
+    hipLaunchKernelGGL((stencil_t2_batch_kernel<TX, TY>), dim3((unsigned)total_tiles), dim3(TX * TY), 0, s, tab, n,
+                       tile_start, lon0, lon1, ta0, ta1, kz, flags, parity);
+    return hipGetLastError();
+}
+
 template <int TX, int TY, bool LUT>
 hipError_t launch_batch_t(const HeatBatchEntry* tab, int n, const int* tile_start, int total_tiles, float lon,
                           double t, int kz, unsigned flags, int parity, hipStream_t s)
@@ -799,8 +844,9 @@ hipError_t launch_batch_t(const HeatBatchEntry* tab, int n, const int* tile_star
 
 int heat_tile_dims(int tile, int* tx, int* ty)
 {
-    static const int d[6][2] = {{64, 4}, {32, 8}, {64, 8}, {64, 4}, {32, 8}, {64, 8}};
-    if (tile < 0 || tile > 5) return -1;
+    // columns owned by one block (the two-step shapes own their footprint's inner columns)
+    static const int d[9][2] = {{64, 4}, {32, 8}, {64, 8}, {64, 4}, {32, 8}, {64, 8}, {30, 6}, {30, 14}, {62, 6}};
+    if (tile < 0 || tile > 8) return -1;
     *tx = d[tile][0];
     *ty = d[tile][1];
     return 0;
@@ -820,7 +866,23 @@ hipError_t launch_heat_batch(const HeatBatchEntry* tab, int n, const int* tile_s
     case 10: return launch_batch_t<64, 8, false>(tab, n, tile_start, total_tiles, lon, time_after, kz, f, parity, s);
     case 11: return launch_batch_t<64, 8, true>(tab, n, tile_start, total_tiles, lon, time_after, kz, f, parity, s);
     }
-    return hipErrorInvalidValue;      // batches use the direct shapes 3-5
+    return hipErrorInvalidValue;      // batches use the direct shapes 3-5 (and launch_heat_batch2)
+}
+
+hipError_t launch_heat_batch2(const HeatBatchEntry* tab, int n, const int* tile_start, int total_tiles, float lon0,
+                              float lon1, double time_after0, double time_after1, int robin, int damage, int energy,
+                              int tile, int kz, int parity, hipStream_t s)
+{
+    const unsigned f = (robin ? F_ROBIN : 0u) | (damage ? F_DAMAGE : 0u) | (energy ? F_ENERGY : 0u);
+    switch (tile) {
+    case 6: return launch_t2_batch_t<32, 8>(tab, n, tile_start, total_tiles, lon0, lon1, time_after0, time_after1, kz,
+                                            f, parity, s);
+    case 7: return launch_t2_batch_t<32, 16>(tab, n, tile_start, total_tiles, lon0, lon1, time_after0, time_after1,
+                                             kz, f, parity, s);
+    case 8: return launch_t2_batch_t<64, 8>(tab, n, tile_start, total_tiles, lon0, lon1, time_after0, time_after1, kz,
+                                            f, parity, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 // launch shapes: 0-2 stage x-y planes in LDS, 3-5 load neighbours directly,

@@ -289,6 +289,10 @@ hipError_t launch_heat_step2(const HeatArgs& a, const float* Tin, float* Tout, f
 long heat_max_blocks(int nx, int ny, int nz);
 // one step of every instance of a table (direct shapes 3-5 only); parity 0
 // reads cur and writes nxt, parity 1 the reverse
+// two steps of every instance of a table (two-step shapes 6-8, class mode)
+hipError_t launch_heat_batch2(const HeatBatchEntry* tab, int n, const int* tile_start, int total_tiles, float lon0,
+                              float lon1, double time_after0, double time_after1, int robin, int damage, int energy,
+                              int tile, int kz, int parity, hipStream_t s);
 hipError_t launch_heat_batch(const HeatBatchEntry* tab, int n, const int* tile_start, int total_tiles, float lon,
                              double time_after, int boil, int robin, int damage, int energy, int lut, int tile,
                              int kz, int parity, hipStream_t s);
