@@ -207,8 +207,9 @@ int tam_heat_run(tam_ctx* ctx, int32_t loops, tam_pulse* pulse);
  * every other step on direct 32x8 (6, 7) or direct 64x8 (8); kz: z planes
  * per block.  Every shape gives bit-identical fields, damage and threshold
  * times.
- * tam_tune_heat times every candidate on the current field (the result goes
- * to the scratch buffer, the state is not advanced) and keeps the fastest. */
+ * tam_tune_heat times every candidate (in class mode the direct and two-step
+ * shapes) x kz on the current field and damage state (results go to scratch
+ * copies, the state is not advanced) and keeps the fastest per step. */
 int tam_set_heat_launch(tam_ctx* ctx, int32_t tile, int32_t kz);
 int tam_tune_heat(tam_ctx* ctx, int32_t reps, int32_t* tile, int32_t* kz, double* step_ms);
 

@@ -132,14 +132,26 @@ def test_class_and_array_modes_agree():
     assert np.array_equal(Tc, Ta)
 
 
-def test_autotune_keeps_state():
+@pytest.mark.parametrize("damage", [0, 1])
+def test_autotune_keeps_state(damage):
     prob, jm = _crater_problem()
+    prob.params.damage_every_step = damage
     g = prob.grid()
     g.upload(lib.FIELD_JMEAN, jm)
-    before = g.download(lib.FIELD_TEMP)
+    if damage:                                  # a damage state with thresholds part crossed
+        T = prob.temp.copy()
+        T[1:-1, 1:-1, 1:-1] = 335.0 + 20.0 * np.random.default_rng(4).random(T[1:-1, 1:-1, 1:-1].shape)
+        g.upload(lib.FIELD_TEMP, T)
+        g.heat_run(5)
+    fields = (lib.FIELD_TEMP, lib.FIELD_TISSUE, lib.FIELD_THRESTIME, lib.FIELD_Q)
+    before = [g.download(f) for f in fields]
+    if damage:
+        assert (before[2] > 0).any() and (before[1] > 0).any()
     tile, kz, ms = g.tune_heat(reps=3)
-    after = g.download(lib.FIELD_TEMP)
+    after = [g.download(f) for f in fields]
+    g.heat_run(3)                               # the live state still steps after tuning
     g.close()
     _print("tuned", lib.__name__, tile, kz, ms)
-    assert 0 <= tile < 9 and kz >= 1 and ms > 0
-    assert np.array_equal(before, after)
+    assert 3 <= tile < 9 and kz >= 1 and ms > 0  # class mode: direct and two-step shapes
+    for f, b, a in zip(fields, before, after):
+        assert np.array_equal(b, a), f

@@ -1227,8 +1227,10 @@ int tam_tune_heat(tam_ctx* c, int32_t reps, int32_t* tile, int32_t* kz, double* 
     if (!c->fp64) {
         // Candidates run the real step (same flags as tam_heat_run) on the
         // current field, cur -> nxt without swapping, so the state does not
-        // advance; the damage integral, threshold times and energy slots they
-        // update are scratch copies.
+        // advance; the damage integral, threshold times, their mask, Q and
+        // the energy slots they update are scratch copies of the live ones,
+        // and the step time is the next step's, so the threshold chain does
+        // the work it does in the run.
         HeatArgs a = heat_args(c);
         const long V = c->nvox;
         double *om = nullptr, *th = nullptr, *eb = nullptr, *qs = nullptr;
@@ -1242,8 +1244,9 @@ int tam_tune_heat(tam_ctx* c, int32_t reps, int32_t* tile, int32_t* kz, double* 
                                                 c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(qs, c->Q, sizeof(double) * (size_t)V, hipMemcpyDeviceToDevice,
                                                 c->stream);
-        if (e == hipSuccess) e = launch_fill_f64(th, 3 * V, 0.0, c->stream);
-        if (e == hipSuccess) e = launch_tmask(th, tm, V, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(th, c->thres, sizeof(double) * 3 * (size_t)V,
+                                                hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(tm, c->tmask, (size_t)V, hipMemcpyDeviceToDevice, c->stream);
         if (e == hipSuccess) e = launch_fill_f64(eb, c->neblk, 0.0, c->stream);
         a.omega = om;
         a.thres = th;
@@ -1253,16 +1256,20 @@ int tam_tune_heat(tam_ctx* c, int32_t reps, int32_t* tile, int32_t* kz, double* 
         const int boil = c->p.boil != 0, robin = c->p.robin_top != 0, dmg = c->p.damage_every_step != 0;
         const int kzs[] = {4, 8, 12, 16, 32, 64};
         const bool two_ok = !boil && c->lut;
+        const double ta = c->time + c->p.delt;
         for (int t = 0; t < heat_num_tiles() && e == hipSuccess; ++t) {
             const bool two = heat_tile_steps(t) == 2;
             if (two && !two_ok) continue;             // would run its one-step twin, timed already
+            // class mode: the LDS-plane shapes (0-2) are slower than their
+            // direct twins on every measured workload; they stay settable
+            if (c->lut && t <= 2) continue;
             for (int kk : kzs) {
                 if (kk > c->p.nz && kk != 4) continue;
                 // a two-step candidate launches two steps, timed per step
                 auto run = [&]() {
-                    return two ? launch_heat_step2(a, c->cur, c->nxt, 1.f, 1.f, 0.0, 0.0, robin, dmg, 1, t, kk,
-                                                   c->stream)
-                               : launch_heat_step(a, c->cur, c->nxt, 1.f, 0.0, boil, robin, dmg, 1, c->lut ? 1 : 0,
+                    return two ? launch_heat_step2(a, c->cur, c->nxt, 1.f, 1.f, ta, ta + c->p.delt, robin, dmg, 1, t,
+                                                   kk, c->stream)
+                               : launch_heat_step(a, c->cur, c->nxt, 1.f, ta, boil, robin, dmg, 1, c->lut ? 1 : 0,
                                                   t, kk, c->stream);
                 };
                 e = run();

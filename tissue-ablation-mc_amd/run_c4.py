@@ -92,6 +92,10 @@ def run(form, prob, args, dn=None, evaluate=False):
                 npk = args.hi
                 cyc["fallback"] = True
         g.normalise(prob.power / (npk * p.volume_voxel))
+        if c in (1, 3, 7, 15):
+            # the damage window fills over the first cycles and the fastest
+            # stencil shape moves with it: re-time the shapes on the state
+            g.tune_heat(reps=3)
         g.heat_run(args.steps)
         g.rebuild()
         g.synchronize()
@@ -99,7 +103,9 @@ def run(form, prob, args, dn=None, evaluate=False):
         T = g.download(lib.FIELD_TEMP)[1:-1, 1:-1, 1:-1]
         m = g.download(lib.FIELD_MAT)
         tissue = m != len(prob.table)
-        cyc.update(npk=npk, max_T_tissue=float(T[tissue].max()), removed=int((~tissue).sum()))
+        tm = g.timing()
+        cyc.update(npk=npk, max_T_tissue=float(T[tissue].max()), removed=int((~tissue).sum()),
+                   heat_ms=tm.heat_ms, heat_shape=[int(tm.tile), int(tm.kz)])
         rec["cycles"].append(cyc)
         log(form, json.dumps(cyc))
         t_eval += time.perf_counter() - t0
