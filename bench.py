@@ -167,14 +167,14 @@ def main():
     stencil["measured_copy_gbps"] = copy_gbps
     stencil["traffic"] = measured_traffic(n, shape)
     # transport: bound by the global FP64 atomic rate of its tally (one add per
-    # voxel visit), whose ceiling is measured on this device.  tally_adds also
-    # counts the deposits that land in the LDS tally box (fresh tissue near the
-    # beam axis: ~1% of this workload's deposits, all in its first cycle)
+    # voxel visit outside the block's LDS tally box), whose ceiling is measured
+    # on this device; deposits into the LDS box are counted apart
     atom_peak = run.grid.atomic_rate()
-    atom = T["tally_adds"] / mc_s
+    atom = (T["tally_adds"] - T["box_adds"]) / mc_s
     mc_ops = (MC_OPS_CROSS * T["crossings"] + MC_OPS_INTER * T["interactions"]) / mc_s
     mc_roof = {"bound": "atomic", "achieved": atom / 1e9, "peak": atom_peak / 1e9, "unit": "Gatomic/s",
                "frac": atom / atom_peak, "tally_adds_per_packet": T["tally_adds"] / T["packets"],
+               "lds_box_adds_per_packet": T["box_adds"] / T["packets"],
                "valu": {"achieved": mc_ops / 1e12, "peak": LANE_OPS_PEAK / 1e12, "unit": "Tlane-op/s",
                         "frac": mc_ops / LANE_OPS_PEAK}}
     roofline = dict(stencil, kernel=f"{kernel} ({shape}, kz {kz})",

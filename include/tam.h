@@ -113,6 +113,7 @@ typedef struct {
     double tally_adds;                               /* FP64 path deposits into the tally, one per
                                                         voxel visit with mu_t > 0 (device atomics,
                                                         or LDS atomics inside the tally box) */
+    double box_adds;                                 /* of these, the ones into the LDS tally box */
 } tam_mc_stats;
 
 /* Pulse state (3dFD.f90:7-12, :199-214); advanced by tam_heat_run. */

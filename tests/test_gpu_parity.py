@@ -132,6 +132,48 @@ def test_mc_launch_shape_does_not_change_results():
         np.testing.assert_allclose(tl, t0, rtol=1e-9, atol=1e-12)
 
 
+def test_mc_box_follows_the_deposition_below_a_crater():
+    """The LDS tally box is placed where the last cycle's deposition (jmean)
+    sums highest under the beam.  Below a deep crater the box sits at the
+    crater floor; box on and off still give the same fates and the same tally
+    up to FP64 summation order, and the oracle's fates.  The case uses a
+    40^3 two-layer grid with a 24-plane crater, an uploaded jmean (the
+    previous run's normalised tally), then a second run."""
+    prob = _two_layer_problem(n=40, packets=200_000)
+    n = 40
+    mua = np.array([t[0] + t[1] for t in prob.table])
+    rk = np.zeros((n + 2,) * 3)
+    rk[1:-1, 1:-1, 1:-1] = mua[prob.mat]
+    zz, yy, xx = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    crater = ((xx - n / 2 + 0.5) ** 2 + (yy - n / 2 + 0.5) ** 2 <= 49) & (zz >= n - 24)
+    rk[1:-1, 1:-1, 1:-1][crater] = 0.0
+    prob.rhokap = rk
+    g = prob.grid()
+    g.set_mc_box(0)
+    g.mc_run(prob.packets, seed=11, stream=0)
+    jm = g.download(lib.FIELD_TALLY)
+    assert jm[: n - 24].sum() > 0.3 * jm.sum()          # much of the deposition below the crater floor
+    g.upload(lib.FIELD_JMEAN, jm)
+    res = []
+    for box in (1, 0, 1):
+        _print(f"case: 40^3 crater 24 planes deep, box {box} placed from the uploaded deposition")
+        g.set_mc_box(box)
+        st = g.mc_run(prob.packets, seed=4321)
+        res.append((st, g.download(lib.FIELD_TALLY)))
+    g.close()
+    st0, t0 = res[0]
+    for st, tl in res[1:]:
+        for k in ("absorbed", "specular", "missed", "crossings", "interactions", "killed", "tally_adds"):
+            assert getattr(st, k) == getattr(st0, k), k
+        np.testing.assert_allclose(tl, t0, rtol=1e-9, atol=1e-12)
+    assert not t0[crater].any()
+    t_or, ost = _oracle_for(prob, 4321).run(prob.packets)
+    fg, fo = st0.fractions(), _fractions_oracle(ost, prob.packets)
+    for k in fg:
+        assert abs(fg[k] - fo[k]) <= 1e-3, k
+    assert np.abs(t0 - t_or).sum() / t_or.sum() < 1e-2
+
+
 def _batches(g, nb, per, seed):
     out = []
     for b in range(nb):

@@ -79,6 +79,9 @@ McArgs mc_args(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream)
     a.blocks_per_cu = c->mc_bpc;
     a.rng_batch = c->mc_batch;
     a.box_enable = c->mc_box;
+    a.box_src = c->jmean;                  // the box follows the last cycle's deposition
+    a.box_psum = c->box_psum;
+    a.box_z = c->box_z;
     return a;
 }
 
@@ -519,6 +522,8 @@ int tam_create(const tam_params* p, tam_ctx** out)
     if (e == hipSuccess) e = dalloc(&c->omega, V);
     if (e == hipSuccess) e = dalloc(&c->thres, 3 * V);
     if (e == hipSuccess) e = dalloc(&c->tmask, V);
+    if (e == hipSuccess) e = dalloc(&c->box_psum, nz);
+    if (e == hipSuccess) e = dalloc(&c->box_z, 1);
     if (e == hipSuccess) e = dalloc(&c->energy, 4);
     if (e == hipSuccess) e = dalloc(&c->flags, 4);
     if (e == hipSuccess) e = dalloc(&c->counters, TAM_CNT_SLOTS);
@@ -579,7 +584,7 @@ void tam_destroy(tam_ctx* c)
                     c->heatcap, c->coeff, c->rhokap, c->mut, c->mat, c->mask, c->mtab, c->mua_frac,
                     c->tally, c->jmean, c->Q, c->W, c->omega, c->thres, c->tmask, c->energy, c->flags,
                     c->counters, c->T64a, c->T64b, c->ax, c->ay, c->az, c->cls, c->glut,
-                    c->robin_lut, c->eblk};
+                    c->robin_lut, c->eblk, c->box_psum, c->box_z};
     for (void* q : ptrs)
         if (q) hipFree(q);
     cycle_reset(c);
@@ -756,6 +761,7 @@ int tam_mc_run(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream, int
         stats->interactions = (double)cnt[TAM_CNT_INTER];
         stats->killed = (double)cnt[TAM_CNT_KILLED];
         stats->tally_adds = (double)cnt[TAM_CNT_TALLY];
+        stats->box_adds = (double)cnt[TAM_CNT_BOX];
         stats->kernel_ms = c->timing.mc_ms;
     }
     return TAM_OK;
@@ -947,6 +953,7 @@ int tam_cycle(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream, doub
     stats->interactions = (double)cnt[TAM_CNT_INTER];
     stats->killed = (double)cnt[TAM_CNT_KILLED];
     stats->tally_adds = (double)cnt[TAM_CNT_TALLY];
+    stats->box_adds = (double)cnt[TAM_CNT_BOX];
     stats->packets = stats->absorbed + stats->specular + stats->missed + stats->killed;
     for (int f = 0; f < 6; ++f) stats->packets += stats->escaped[f];
     return rc;
@@ -1529,6 +1536,7 @@ int tam_batch_mc_run(tam_batch* b, const int64_t* npackets, uint64_t seed, uint3
         s.interactions = (double)cnt[TAM_CNT_INTER];
         s.killed = (double)cnt[TAM_CNT_KILLED];
         s.tally_adds = (double)cnt[TAM_CNT_TALLY];
+        s.box_adds = (double)cnt[TAM_CNT_BOX];
         s.kernel_ms = b->last_ms;
     }
     return TAM_OK;

@@ -116,25 +116,28 @@ enum : int { PEND_NONE = 0, PEND_LAUNCH = 1, PEND_INTERACT = 2, PEND_FRESNEL = 3
 
 // The transport of one instance by the calling wave's lanes, until its
 // packet queue is drained; the fate counters go to a.counters.
-// Path length into the tally: voxels inside the block's LDS box are summed
-// there (an LDS FP64 atomic), the rest go to the device tally directly.
+// Path length into the tally: voxels inside the block's LDS box (z origin
+// boz) are summed there (an LDS FP64 atomic), the rest go to the device
+// tally directly.  Returns 1 for a deposit into the box.
 template <bool BOX>
-__device__ __forceinline__ void tally_add(const McArgs& a, double* box, uint32_t v, int ci, int cj, int ck, float l)
+__device__ __forceinline__ uint32_t tally_add(const McArgs& a, double* box, int boz, uint32_t v, int ci, int cj,
+                                              int ck, float l)
 {
     if (BOX) {
         const unsigned bx = (unsigned)(ci - a.box_o[0]), by = (unsigned)(cj - a.box_o[1]),
-                       bz = (unsigned)(ck - a.box_o[2]);
+                       bz = (unsigned)(ck - boz);
         if (bx < (unsigned)a.box_n[0] && by < (unsigned)a.box_n[1] && bz < (unsigned)a.box_n[2]) {
             __hip_atomic_fetch_add(&box[bx + (unsigned)a.box_n[0] * (by + (unsigned)a.box_n[1] * bz)], (double)l,
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            return;
+            return 1u;
         }
     }
     unsafeAtomicAdd(&a.tally[v], (double)l);
+    return 0u;
 }
 
 template <bool BOX>
-__device__ __forceinline__ void transport(const McArgs& a, double* box)
+__device__ __forceinline__ void transport(const McArgs& a, double* box, int boz)
 {
     unsigned long long* head = a.counters + TAM_CNT_N;     // queue head, packets taken
     const uint32_t nxy = (uint32_t)a.nx * (uint32_t)a.ny;
@@ -144,7 +147,7 @@ __device__ __forceinline__ void transport(const McArgs& a, double* box)
     const uint64_t npk = a.dyn_npk ? (uint64_t)*a.dyn_npk : a.npk;
     const uint32_t stream = a.dyn_stream ? *a.dyn_stream : a.stream;
 
-    uint32_t c_abs = 0, c_spec = 0, c_miss = 0, c_cross = 0, c_inter = 0, c_kill = 0, c_tally = 0;
+    uint32_t c_abs = 0, c_spec = 0, c_miss = 0, c_cross = 0, c_inter = 0, c_kill = 0, c_tally = 0, c_box = 0;
     uint32_t c_esc[6] = {0, 0, 0, 0, 0, 0};
 
     // wave-uniform queue state: packets [pool, pool_end) are this wave's
@@ -256,7 +259,7 @@ __device__ __forceinline__ void transport(const McArgs& a, double* box)
                     ++c_inter;
                     const MatEntry me = a.mtab[mat];
                     if (!(u01(r.x) < me.albedo)) {
-                        tally_add<BOX>(a, box, v, ci, cj, ck, lacc);
+                        c_box += tally_add<BOX>(a, box, boz, v, ci, cj, ck, lacc);
                         ++c_tally;
                         lacc = 0.f;
                         ++c_abs;
@@ -338,7 +341,7 @@ __device__ __forceinline__ void transport(const McArgs& a, double* box)
                 // mu_t = 0 (ablated): nothing is absorbed, and the atomic
                 // unit's throughput is what bounds this kernel
                 if (mu > 0.f) {
-                    tally_add<BOX>(a, box, v, ci, cj, ck, lacc);
+                    c_box += tally_add<BOX>(a, box, boz, v, ci, cj, ck, lacc);
                     ++c_tally;
                 }
                 lacc = 0.f;
@@ -384,7 +387,7 @@ __device__ __forceinline__ void transport(const McArgs& a, double* box)
     // runs once every lane of the wave has left)
     const uint32_t vals[TAM_CNT_N] = {c_abs, c_spec, c_esc[0], c_esc[1], c_esc[2], c_esc[3],
                                       c_esc[4], c_esc[5], c_miss, c_cross, c_inter, c_kill,
-                                      c_tally};
+                                      c_tally, c_box};
     const bool leader = (threadIdx.x & 63) == 0;
 #pragma unroll
     for (int i = 0; i < TAM_CNT_N; ++i) {
@@ -393,7 +396,7 @@ __device__ __forceinline__ void transport(const McArgs& a, double* box)
     }
 }
 
-__global__ __launch_bounds__(256) void mc_kernel(McArgs a) { transport<false>(a, nullptr); }
+__global__ __launch_bounds__(256) void mc_kernel(McArgs a) { transport<false>(a, nullptr, 0); }
 
 // The same with the block-private LDS tally box: cleared first, added to the
 // device tally (one atomic per touched voxel) once every wave of the block
@@ -401,10 +404,12 @@ __global__ __launch_bounds__(256) void mc_kernel(McArgs a) { transport<false>(a,
 __global__ __launch_bounds__(256) void mc_box_kernel(McArgs a)
 {
     extern __shared__ double box[];
+    // z origin: placed by box_window_kernel, in [0, nz - box_n[2]]
+    const int boz = a.box_z ? __builtin_amdgcn_readfirstlane(*a.box_z) : a.box_o[2];
     const int nbox = a.box_n[0] * a.box_n[1] * a.box_n[2];
     for (int i = threadIdx.x; i < nbox; i += 256) box[i] = 0.0;
     __syncthreads();
-    transport<true>(a, box);
+    transport<true>(a, box, boz);
     __syncthreads();
     const uint32_t nxy = (uint32_t)a.nx * (uint32_t)a.ny;
     for (int i = threadIdx.x; i < nbox; i += 256) {
@@ -412,10 +417,49 @@ __global__ __launch_bounds__(256) void mc_box_kernel(McArgs a)
         if (s != 0.0) {
             const int bx = i % a.box_n[0], by = (i / a.box_n[0]) % a.box_n[1], bz = i / (a.box_n[0] * a.box_n[1]);
             const uint32_t v = (uint32_t)(a.box_o[0] + bx) + (uint32_t)a.nx * (uint32_t)(a.box_o[1] + by) +
-                               nxy * (uint32_t)(a.box_o[2] + bz);
+                               nxy * (uint32_t)(boz + bz);
             unsafeAtomicAdd(&a.tally[v], s);
         }
     }
+}
+
+// Placement of the LDS tally box in z.  box_plane_kernel: one wave per z
+// plane sums the deposition field over the box's x-y footprint, skipping
+// voxels removed since (mu_t = 0: ablated after that deposition);
+// box_window_kernel: one lane picks the d consecutive planes with the largest
+// sum (ties: the higher planes; no deposition at all: the top planes).
+__global__ __launch_bounds__(64) void box_plane_kernel(const double* __restrict__ src, const float* __restrict__ mut,
+                                                       int nx, int ny, int ox, int oy, int sx, int sy,
+                                                       double* __restrict__ psum)
+{
+    const int k = blockIdx.x;
+    double t = 0.0;
+    for (int q = threadIdx.x; q < sx * sy; q += 64) {
+        const int i = ox + q % sx, j = oy + q / sx;
+        const long v = (long)i + (long)nx * (j + (long)ny * k);
+        if (mut[v] > 0.f) t += src[v];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if (threadIdx.x == 0) psum[k] = t;
+}
+
+__global__ __launch_bounds__(64) void box_window_kernel(const double* __restrict__ psum, int nz, int d,
+                                                        int* __restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    double w = 0.0;
+    for (int k = nz - d; k < nz; ++k) w += psum[k];
+    double best = w;
+    int at = nz - d;
+    for (int k = nz - d - 1; k >= 0; --k) {       // window [k, k + d)
+        w += psum[k] - psum[k + d];
+        if (w > best) {
+            best = w;
+            at = k;
+        }
+    }
+    out[0] = at;
 }
 
 // Many instances in one launch: a wave takes packets from one instance's
@@ -430,7 +474,7 @@ __global__ __launch_bounds__(256) void mc_batch_kernel(const McArgs* __restrict_
         const int i = (wave + k) % n;
         const McArgs a = tab[i];
         if (a.npk == 0) continue;
-        transport<false>(a, nullptr);
+        transport<false>(a, nullptr, 0);
     }
 }
 
@@ -488,13 +532,16 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
     const long need = (long)((a.npk + 255) / 256);
     if (need < blocks) blocks = need;
     // LDS tally box: the resident blocks of a CU share its 160 KiB of LDS;
-    // at most 64 KiB (8192 voxels) per block.  An s x s x d box on the beam
-    // axis from the top face down (d ~ s / 2), where most of the path of a
-    // surface beam lies.
+    // at most 64 KiB (8192 voxels) per block.  A cube (s x s x d, d ~ s) on
+    // the beam axis, by default from the top face down, where most of the
+    // path of a surface beam lies; with a deposition field it moves down to
+    // where that field is largest.  On the c1 skin cycles a cube caught more
+    // deposits past the first cycle than flatter (d ~ s/2, s/8) or taller
+    // (d ~ 2s) boxes of the same volume.
     McArgs b = a;
     b.box_n[0] = b.box_n[1] = b.box_n[2] = 0;
     const long vox = std::min(8192L, (160L * 1024L / bpc - 1024L) / 8L);
-    int sd = (int)std::cbrt(2.0 * (double)vox);
+    int sd = (int)std::cbrt((double)vox);
     sd = std::min(sd, std::min(a.nx, a.ny));
     const int d = sd > 0 ? std::min<long>(a.nz, vox / ((long)sd * sd)) : 0;
     if (a.box_enable && sd >= 4 && d >= 1) {
@@ -505,6 +552,16 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
         b.box_n[0] = sd;
         b.box_n[1] = sd;
         b.box_n[2] = d;
+        b.box_z = nullptr;
+        if (a.box_src && a.box_psum && a.box_z) {
+            // depth: where the deposition field (the last cycle's) is largest
+            // under the beam in the tissue still present; after ablation that
+            // is below the crater floor
+            hipLaunchKernelGGL(box_plane_kernel, dim3((unsigned)a.nz), dim3(64), 0, s, a.box_src, a.mut, a.nx, a.ny,
+                               b.box_o[0], b.box_o[1], sd, sd, a.box_psum);
+            hipLaunchKernelGGL(box_window_kernel, dim3(1), dim3(64), 0, s, a.box_psum, a.nz, d, a.box_z);
+            b.box_z = a.box_z;
+        }
         const size_t lds = sizeof(double) * (size_t)sd * (size_t)sd * (size_t)d;
         hipLaunchKernelGGL(mc_box_kernel, dim3((unsigned)blocks), dim3(256), lds, s, b);
     } else {

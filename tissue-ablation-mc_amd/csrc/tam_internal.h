@@ -79,11 +79,18 @@ struct McArgs {
     // block-private FP64 tally box in LDS (box_n[0] == 0: none): voxels
     // [box_o, box_o + box_n) are summed per block and added to the tally once
     int box_o[3], box_n[3];
+    // box depth from a deposition field: launch_mc places the box's z range
+    // where box_src (interior, e.g. the last cycle's jmean) sums highest
+    // inside the box's x-y footprint, through box_psum (nz doubles) into
+    // *box_z, which the box kernel reads (nullptr: box_o[2] as given)
+    const double* box_src;
+    double* box_psum;
+    int* box_z;
 };
 
 // counters[TAM_CNT_N] is the transport's packet queue head
 enum { TAM_CNT_ABSORBED = 0, TAM_CNT_SPECULAR, TAM_CNT_ESC0, TAM_CNT_MISSED = TAM_CNT_ESC0 + 6,
-       TAM_CNT_CROSS, TAM_CNT_INTER, TAM_CNT_KILLED, TAM_CNT_TALLY, TAM_CNT_N, TAM_CNT_SLOTS };
+       TAM_CNT_CROSS, TAM_CNT_INTER, TAM_CNT_KILLED, TAM_CNT_TALLY, TAM_CNT_BOX, TAM_CNT_N, TAM_CNT_SLOTS };
 
 struct HeatArgs {
     int nx, ny, nz;
@@ -233,6 +240,8 @@ struct tam_ctx {
     double* mua_frac;           // per material mu_a / mu_t (FP64)
     double *tally, *jmean, *Q, *W, *omega, *thres;
     uint8_t* tmask;             // bit k: thres[k] != 0 (kept with every write of thres)
+    double* box_psum;           // LDS tally box placement: per-plane sums (nz) and the chosen z origin
+    int* box_z;
     double* energy;             // [0] boundary outflow, [1..] reduction scratch
     int* flags;
     unsigned long long* counters;

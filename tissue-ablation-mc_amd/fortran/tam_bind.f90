@@ -42,7 +42,7 @@ module tam_bind
 
     type, bind(C) :: tam_mc_stats
         real(c_double) :: packets, absorbed, specular, escaped(6), missed
-        real(c_double) :: crossings, interactions, kernel_ms, killed, tally_adds
+        real(c_double) :: crossings, interactions, kernel_ms, killed, tally_adds, box_adds
     end type tam_mc_stats
 
     type, bind(C) :: tam_pulse

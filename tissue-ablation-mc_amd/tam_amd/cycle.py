@@ -21,7 +21,8 @@ class CycleRunner:
         self.tuned_mc = self.grid.tune_mc(min(int(prob.packets), 5_000_000)) if tune else None
         self.time = 0.0
         self.cycle = 0
-        self.totals = dict(packets=0.0, crossings=0.0, interactions=0.0, tally_adds=0.0, mc_ms=0.0, heat_ms=0.0,
+        self.totals = dict(packets=0.0, crossings=0.0, interactions=0.0, tally_adds=0.0, box_adds=0.0,
+                           mc_ms=0.0, heat_ms=0.0,
                            heat_steps=0, rebuild_ms=0.0, normalise_ms=0.0, arrhenius_ms=0.0,
                            voxel_updates=0.0)
         self.heat_by_shape = {}      # (tile, kz) -> [heat ms, steps] of the stepwise cycles
@@ -60,6 +61,7 @@ class CycleRunner:
         T["crossings"] += st.crossings
         T["interactions"] += st.interactions
         T["tally_adds"] += st.tally_adds
+        T["box_adds"] += st.box_adds
         T["mc_ms"] += st.kernel_ms
         T["heat_ms"] += t.heat_ms
         T["heat_steps"] += t.heat_steps
@@ -86,6 +88,7 @@ class CycleRunner:
         T["crossings"] += st.crossings
         T["interactions"] += st.interactions
         T["tally_adds"] += st.tally_adds
+        T["box_adds"] += st.box_adds
         T["cycle_ms"] = T.get("cycle_ms", 0.0) + t.cycle_ms
         T["voxel_updates"] += float(p.steps) * p.params.nx * p.params.ny * p.params.nz
         self.cycle += 1

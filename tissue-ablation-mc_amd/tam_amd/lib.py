@@ -61,7 +61,7 @@ class MCStats(C.Structure):
     _fields_ = [("packets", C.c_double), ("absorbed", C.c_double), ("specular", C.c_double),
                 ("escaped", C.c_double * 6), ("missed", C.c_double), ("crossings", C.c_double),
                 ("interactions", C.c_double), ("kernel_ms", C.c_double), ("killed", C.c_double),
-                ("tally_adds", C.c_double)]
+                ("tally_adds", C.c_double), ("box_adds", C.c_double)]
 
     def fractions(self):
         n = self.packets
