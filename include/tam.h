@@ -179,10 +179,13 @@ int tam_mc_run(tam_ctx* ctx, int64_t npackets, uint64_t seed, uint32_t stream, i
  * the fastest (the tally is cleared afterwards), with the LDS tally box on
  * and off.  None of them changes results beyond FP64 summation order:
  * packet p always draws from its own stream.
- * tam_set_mc_box: each block sums the path it lays in a box of voxels on the
- * beam axis below the top face (at most 64 KiB of FP64, sized to the
- * blocks per CU) in LDS and adds the box to the tally once at the end,
- * instead of one device atomic per voxel visit (1 = on, the default). */
+ * tam_set_mc_box: each block sums the path it lays in a cube of voxels on
+ * the beam axis (at most 64 KiB of FP64, sized to the blocks per CU) in LDS
+ * and adds the box to the tally once at the end, instead of one device
+ * atomic per voxel visit (1 = on, the default).  The cube's z range is
+ * where the current jmean (the last normalised deposition), over voxels
+ * with mu_t > 0, sums highest inside its x-y footprint; with jmean zero it
+ * starts at the top face. */
 int tam_set_mc_launch(tam_ctx* ctx, int32_t blocks_per_cu, int32_t rng_batch);
 int tam_set_mc_box(tam_ctx* ctx, int32_t enable);
 int tam_tune_mc(tam_ctx* ctx, int64_t sample, int32_t* blocks_per_cu, int32_t* rng_batch,
