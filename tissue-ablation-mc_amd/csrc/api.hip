@@ -562,6 +562,7 @@ int tam_create(const tam_params* p, tam_ctx** out)
     e = hipMemsetAsync(c->mat, 0, (size_t)V, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = mc_prepare(c->dev);
     if (e != hipSuccess) {
         std::string m = std::string("device init: ") + hipGetErrorString(e);
         tam_destroy(c);
@@ -1183,7 +1184,7 @@ int tam_tune_mc(tam_ctx* c, int64_t sample, int32_t* blocks_per_cu, int32_t* rng
     if (!c) return TAM_ERR_ARG;
     if (sample < 1) return fail(c, TAM_ERR_ARG, "sample must be positive");
     const int old_bpc = c->mc_bpc, old_batch = c->mc_batch, old_box = c->mc_box;
-    const int bpcs[] = {2, 4, 6, 8};
+    const int bpcs[] = {1, 2, 3, 4, 6, 8};
     const int batches[] = {8, 16, 24, 32, 40, 48};
     double best = 0.0;
     int bb = c->mc_bpc, bt = c->mc_batch, bx = c->mc_box;

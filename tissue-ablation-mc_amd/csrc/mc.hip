@@ -517,13 +517,29 @@ hipError_t launch_mc_batch(const McArgs* dtab, int n, long total_npk, int blocks
     return hipGetLastError();
 }
 
+hipError_t mc_prepare(int dev)
+{
+    int lds_blk = 0;
+    hipError_t e = hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess || lds_blk <= 64 * 1024) return e;
+    // allow the box kernel the device's whole per-block LDS (outside any graph capture)
+    return hipFuncSetAttribute((const void*)mc_box_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_blk);
+}
+
 hipError_t launch_mc(const McArgs& a, hipStream_t s)
 {
     if (a.npk == 0) return hipSuccess;
-    int dev = 0, ncu = 0;
+    int dev = 0, ncu = 0, lds_cu = 0, lds_blk = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess || ncu <= 0) ncu = 256;
+    // LDS per CU and per block (160 KiB each on MI355X)
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+        lds_cu <= 0)
+        lds_cu = 64 * 1024;
+    if (hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        lds_blk <= 0)
+        lds_blk = 64 * 1024;
     // blocks_per_cu 256-lane blocks per CU (autotuned); every lane takes
     // packets from the queue until it is empty, so more blocks than packets
     // only idle
@@ -531,8 +547,10 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
     long blocks = (long)ncu * bpc;
     const long need = (long)((a.npk + 255) / 256);
     if (need < blocks) blocks = need;
-    // LDS tally box: the resident blocks of a CU share its 160 KiB of LDS;
-    // at most 64 KiB (8192 voxels) per block.  A cube (s x s x d, d ~ s) on
+    // LDS tally box: the resident blocks of a CU share its LDS (160 KiB on
+    // MI355X, also the most one block may take), 1 KiB per block kept for
+    // the kernel's own use: 27^3 voxels at 1 block per CU, 21^3 at 2, 17^3
+    // at 4.  A cube (s x s x d, d ~ s) on
     // the beam axis, by default from the top face down, where most of the
     // path of a surface beam lies; with a deposition field it moves down to
     // where that field is largest.  On the c1 skin cycles a cube caught more
@@ -540,7 +558,7 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
     // (d ~ 2s) boxes of the same volume.
     McArgs b = a;
     b.box_n[0] = b.box_n[1] = b.box_n[2] = 0;
-    const long vox = std::min(8192L, (160L * 1024L / bpc - 1024L) / 8L);
+    const long vox = std::min((long)lds_blk - 1024L, (long)lds_cu / bpc - 1024L) / 8L;
     int sd = (int)std::cbrt((double)vox);
     sd = std::min(sd, std::min(a.nx, a.ny));
     const int d = sd > 0 ? std::min<long>(a.nz, vox / ((long)sd * sd)) : 0;
