@@ -574,25 +574,33 @@ __device__ __forceinline__ void t2_body(const HeatArgs& a, const float* __restri
     float Gxm = 0.f, Gxp = 0.f, Gym = 0.f, Gyp = 0.f, Gzm = 0.f, Gzp = 0.f, Sk = 0.f;
     int Ck = 0;
     float e_acc = 0.f;
+    // loads of plane q (x-y neighbours, source) and of the column at q+1,
+    // issued one iteration ahead of their use
+    float n_z0p = 0.f, n_txm = 0.f, n_txp = 0.f, n_tym = 0.f, n_typ = 0.f, n_s = 0.f;
+    unsigned n_cpr = 0;
+    auto issue = [&](int q) {
+        if (q <= nz) {                                        // plane q+1 <= nz+1 exists
+            n_z0p = Tin[ch + pxy * (q + 1)];
+            n_cpr = cla[ch + pxy * (q + 1)];
+        }
+        if (q >= 1 && q <= nz) {
+            const long c = ci + pxy * q;
+            n_txm = Tin[c - 1];
+            n_txp = Tin[c + 1];
+            n_tym = Tin[c - px];
+            n_typ = Tin[c + px];
+            n_s = srca[c];
+        }
+    };
+    issue(p);
     for (; p <= k1; ++p) {
         const bool interior = p >= 1 && p <= nz;              // else a halo plane: dT fixed
         const int k = p - 1;
         const bool step2 = p > k0;                            // second step at k in this iteration
-        // ---- loads of plane p (and the column at p+1); Omega of (i, j, k)
-        float z0p = 0.f, txm = 0.f, txp = 0.f, tym = 0.f, typ = 0.f, s_p = 0.f;
-        unsigned cpr = 0;
-        if (p <= nz) {                                        // plane p+1 <= nz+1 exists
-            z0p = Tin[ch + pxy * (p + 1)];
-            cpr = cla[ch + pxy * (p + 1)];
-        }
-        if (interior) {
-            const long c = ci + pxy * p;
-            txm = Tin[c - 1];
-            txp = Tin[c + 1];
-            tym = Tin[c - px];
-            typ = Tin[c + px];
-            s_p = srca[c];
-        }
+        // ---- plane p's values (loaded last iteration); plane p+1's loads go out now
+        const float z0p = n_z0p, txm = n_txm, txp = n_txp, tym = n_tym, typ = n_typ, s_p = n_s;
+        const unsigned cpr = n_cpr;
+        if (p < k1) issue(p + 1);
         const long v = vcol + nxy * (k - 1);
         const bool pom = step2 && damage && own && u1c >= wlo && u1c < whi;
         double om_pf = 0.0;
