@@ -25,17 +25,23 @@ def _skin(steps):
     return prob
 
 
-# (steps per cycle, stepwise launch shape, graph launch shape); shape 8 runs
-# two steps per launch, 4 one (None: the default shape)
-@pytest.mark.parametrize("steps,shape_a,shape_b", [(60, None, None), (61, None, None), (61, 4, 8), (60, 8, 4)])
-def test_graph_cycle_matches_stepwise_fp32(steps, shape_a, shape_b):
-    _print(f"case: 40^3 skin, {steps} steps per cycle, 4 cycles, graph vs stepwise, shapes {shape_a} {shape_b}")
+# (steps per cycle, stepwise launch shape, graph launch shape, transport
+# blocks per CU); shape 8 runs two steps per launch, 4 one (None: the
+# default shape); 1 block per CU gives the largest LDS tally box (160 KiB)
+@pytest.mark.parametrize("steps,shape_a,shape_b,bpc", [(60, None, None, None), (61, None, None, None),
+                                                       (61, 4, 8, None), (60, 8, 4, 1)])
+def test_graph_cycle_matches_stepwise_fp32(steps, shape_a, shape_b, bpc):
+    _print(f"case: 40^3 skin, {steps} steps per cycle, 4 cycles, graph vs stepwise, shapes {shape_a} {shape_b}, "
+           f"transport blocks per CU {bpc}")
     prob = _skin(steps)
     a = CycleRunner(prob, seed=7, tune=False, graph=False)
     b = CycleRunner(prob, seed=7, tune=False, graph=True)
     if shape_a is not None:
         a.grid.set_heat_launch(shape_a, 8)
         b.grid.set_heat_launch(shape_b, 8)
+    if bpc is not None:
+        a.grid.set_mc_launch(bpc, 32)
+        b.grid.set_mc_launch(bpc, 32)
     for cyc in range(4):
         sa = a.run_cycle()
         sb = b.run_cycle()

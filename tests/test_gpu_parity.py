@@ -133,12 +133,12 @@ def test_mc_launch_shape_does_not_change_results():
 
 
 def test_mc_box_follows_the_deposition_below_a_crater():
-    """The LDS tally box is placed where the last cycle's deposition (jmean)
-    sums highest under the beam.  Below a deep crater the box sits at the
-    crater floor; box on and off still give the same fates and the same tally
-    up to FP64 summation order, and the oracle's fates.  The case uses a
-    40^3 two-layer grid with a 24-plane crater, an uploaded jmean (the
-    previous run's normalised tally), then a second run."""
+    """The LDS tally box takes the shape and z range that held the most of
+    the previous run's path.  Below a deep crater that is the crater floor;
+    box on and off still give the same fates and the same tally up to FP64
+    summation order, and the oracle's fates.  The case is a 40^3 two-layer
+    grid with a 24-plane crater: one run with the box off, then runs that
+    place the box from the run before."""
     prob = _two_layer_problem(n=40, packets=200_000)
     n = 40
     mua = np.array([t[0] + t[1] for t in prob.table])
@@ -152,11 +152,10 @@ def test_mc_box_follows_the_deposition_below_a_crater():
     g.set_mc_box(0)
     g.mc_run(prob.packets, seed=11, stream=0)
     jm = g.download(lib.FIELD_TALLY)
-    assert jm[: n - 24].sum() > 0.3 * jm.sum()          # much of the deposition below the crater floor
-    g.upload(lib.FIELD_JMEAN, jm)
+    assert jm[: n - 24].sum() > 0.3 * jm.sum()          # much of the path below the crater floor
     res = []
     for box in (1, 0, 1):
-        _print(f"case: 40^3 crater 24 planes deep, box {box} placed from the uploaded deposition")
+        _print(f"case: 40^3 crater 24 planes deep, box {box} placed from the previous run's path")
         g.set_mc_box(box)
         st = g.mc_run(prob.packets, seed=4321)
         res.append((st, g.download(lib.FIELD_TALLY)))

@@ -79,9 +79,8 @@ McArgs mc_args(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream)
     a.blocks_per_cu = c->mc_bpc;
     a.rng_batch = c->mc_batch;
     a.box_enable = c->mc_box;
-    a.box_src = c->jmean;                  // the box follows the last cycle's deposition
-    a.box_psum = c->box_psum;
-    a.box_z = c->box_z;
+    a.box_psum = c->box_psum;              // the box follows the last run's path
+    a.box_geom = c->box_geom;
     return a;
 }
 
@@ -340,8 +339,8 @@ int enqueue_cycle(tam_ctx* c)
     hipStream_t s = c->stream;
     // transport (mcpolar.f90:151-170) and normalisation (:173-174)
     HIPCHK(c, launch_fill_u64(c->counters, TAM_CNT_SLOTS, s));
-    HIPCHK(c, launch_fill_f64(c->tally, c->nvox, 0.0, s));
     McArgs m = mc_args(c, c->cyc_npk, c->cyc_seed, 0);
+    m.clear_tally = 1;                     // after the LDS-box placement has read the last cycle's tally
     m.dyn_npk = &hdr->npk;
     m.dyn_stream = &hdr->stream;
     HIPCHK(c, launch_mc(m, s));
@@ -523,7 +522,7 @@ int tam_create(const tam_params* p, tam_ctx** out)
     if (e == hipSuccess) e = dalloc(&c->thres, 3 * V);
     if (e == hipSuccess) e = dalloc(&c->tmask, V);
     if (e == hipSuccess) e = dalloc(&c->box_psum, nz);
-    if (e == hipSuccess) e = dalloc(&c->box_z, 1);
+    if (e == hipSuccess) e = dalloc(&c->box_geom, 6);
     if (e == hipSuccess) e = dalloc(&c->energy, 4);
     if (e == hipSuccess) e = dalloc(&c->flags, 4);
     if (e == hipSuccess) e = dalloc(&c->counters, TAM_CNT_SLOTS);
@@ -585,7 +584,7 @@ void tam_destroy(tam_ctx* c)
                     c->heatcap, c->coeff, c->rhokap, c->mut, c->mat, c->mask, c->mtab, c->mua_frac,
                     c->tally, c->jmean, c->Q, c->W, c->omega, c->thres, c->tmask, c->energy, c->flags,
                     c->counters, c->T64a, c->T64b, c->ax, c->ay, c->az, c->cls, c->glut,
-                    c->robin_lut, c->eblk, c->box_psum, c->box_z};
+                    c->robin_lut, c->eblk, c->box_psum, c->box_geom};
     for (void* q : ptrs)
         if (q) hipFree(q);
     cycle_reset(c);
@@ -740,9 +739,9 @@ int tam_mc_run(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream, int
     if (npackets < 0) return fail(c, TAM_ERR_ARG, "negative packet count");
     if (!c->optics_set) return fail(c, TAM_ERR_STATE, "tam_set_optics must come first");
     Dev d(c->dev);
-    if (!accumulate) HIPCHK(c, launch_fill_f64(c->tally, c->nvox, 0.0, c->stream));
     HIPCHK(c, launch_fill_u64(c->counters, TAM_CNT_SLOTS, c->stream));
-    const McArgs a = mc_args(c, npackets, seed, stream);
+    McArgs a = mc_args(c, npackets, seed, stream);
+    a.clear_tally = accumulate ? 0 : 1;    // after the LDS-box placement has read the last run's tally
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, launch_mc(a, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));

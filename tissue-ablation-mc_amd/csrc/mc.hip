@@ -116,18 +116,22 @@ enum : int { PEND_NONE = 0, PEND_LAUNCH = 1, PEND_INTERACT = 2, PEND_FRESNEL = 3
 
 // The transport of one instance by the calling wave's lanes, until its
 // packet queue is drained; the fate counters go to a.counters.
-// Path length into the tally: voxels inside the block's LDS box (z origin
-// boz) are summed there (an LDS FP64 atomic), the rest go to the device
-// tally directly.  Returns 1 for a deposit into the box.
+// The LDS tally box of a launch: voxels [o, o + n).
+struct BoxGeom {
+    int o[3], n[3];
+};
+
+// Path length into the tally: voxels inside the block's LDS box are summed
+// there (an LDS FP64 atomic), the rest go to the device tally directly.
+// Returns 1 for a deposit into the box.
 template <bool BOX>
-__device__ __forceinline__ uint32_t tally_add(const McArgs& a, double* box, int boz, uint32_t v, int ci, int cj,
-                                              int ck, float l)
+__device__ __forceinline__ uint32_t tally_add(const McArgs& a, double* box, const BoxGeom& g, uint32_t v, int ci,
+                                              int cj, int ck, float l)
 {
     if (BOX) {
-        const unsigned bx = (unsigned)(ci - a.box_o[0]), by = (unsigned)(cj - a.box_o[1]),
-                       bz = (unsigned)(ck - boz);
-        if (bx < (unsigned)a.box_n[0] && by < (unsigned)a.box_n[1] && bz < (unsigned)a.box_n[2]) {
-            __hip_atomic_fetch_add(&box[bx + (unsigned)a.box_n[0] * (by + (unsigned)a.box_n[1] * bz)], (double)l,
+        const unsigned bx = (unsigned)(ci - g.o[0]), by = (unsigned)(cj - g.o[1]), bz = (unsigned)(ck - g.o[2]);
+        if (bx < (unsigned)g.n[0] && by < (unsigned)g.n[1] && bz < (unsigned)g.n[2]) {
+            __hip_atomic_fetch_add(&box[bx + (unsigned)g.n[0] * (by + (unsigned)g.n[1] * bz)], (double)l,
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             return 1u;
         }
@@ -137,7 +141,7 @@ __device__ __forceinline__ uint32_t tally_add(const McArgs& a, double* box, int 
 }
 
 template <bool BOX>
-__device__ __forceinline__ void transport(const McArgs& a, double* box, int boz)
+__device__ __forceinline__ void transport(const McArgs& a, double* box, const BoxGeom& g)
 {
     unsigned long long* head = a.counters + TAM_CNT_N;     // queue head, packets taken
     const uint32_t nxy = (uint32_t)a.nx * (uint32_t)a.ny;
@@ -259,7 +263,7 @@ __device__ __forceinline__ void transport(const McArgs& a, double* box, int boz)
                     ++c_inter;
                     const MatEntry me = a.mtab[mat];
                     if (!(u01(r.x) < me.albedo)) {
-                        c_box += tally_add<BOX>(a, box, boz, v, ci, cj, ck, lacc);
+                        c_box += tally_add<BOX>(a, box, g, v, ci, cj, ck, lacc);
                         ++c_tally;
                         lacc = 0.f;
                         ++c_abs;
@@ -341,7 +345,7 @@ __device__ __forceinline__ void transport(const McArgs& a, double* box, int boz)
                 // mu_t = 0 (ablated): nothing is absorbed, and the atomic
                 // unit's throughput is what bounds this kernel
                 if (mu > 0.f) {
-                    c_box += tally_add<BOX>(a, box, boz, v, ci, cj, ck, lacc);
+                    c_box += tally_add<BOX>(a, box, g, v, ci, cj, ck, lacc);
                     ++c_tally;
                 }
                 lacc = 0.f;
@@ -396,7 +400,7 @@ __device__ __forceinline__ void transport(const McArgs& a, double* box, int boz)
     }
 }
 
-__global__ __launch_bounds__(256) void mc_kernel(McArgs a) { transport<false>(a, nullptr, 0); }
+__global__ __launch_bounds__(256) void mc_kernel(McArgs a) { transport<false>(a, nullptr, BoxGeom{}); }
 
 // The same with the block-private LDS tally box: cleared first, added to the
 // device tally (one atomic per touched voxel) once every wave of the block
@@ -404,62 +408,107 @@ __global__ __launch_bounds__(256) void mc_kernel(McArgs a) { transport<false>(a,
 __global__ __launch_bounds__(256) void mc_box_kernel(McArgs a)
 {
     extern __shared__ double box[];
-    // z origin: placed by box_window_kernel, in [0, nz - box_n[2]]
-    const int boz = a.box_z ? __builtin_amdgcn_readfirstlane(*a.box_z) : a.box_o[2];
-    const int nbox = a.box_n[0] * a.box_n[1] * a.box_n[2];
+    // geometry: chosen by box_window_kernel (inside the grid, at most the
+    // LDS this launch was given), or the launch's default
+    BoxGeom g;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        g.o[q] = a.box_geom ? __builtin_amdgcn_readfirstlane(a.box_geom[q]) : a.box_o[q];
+        g.n[q] = a.box_geom ? __builtin_amdgcn_readfirstlane(a.box_geom[3 + q]) : a.box_n[q];
+    }
+    const int nbox = g.n[0] * g.n[1] * g.n[2];
     for (int i = threadIdx.x; i < nbox; i += 256) box[i] = 0.0;
     __syncthreads();
-    transport<true>(a, box, boz);
+    transport<true>(a, box, g);
     __syncthreads();
     const uint32_t nxy = (uint32_t)a.nx * (uint32_t)a.ny;
     for (int i = threadIdx.x; i < nbox; i += 256) {
         const double s = box[i];
         if (s != 0.0) {
-            const int bx = i % a.box_n[0], by = (i / a.box_n[0]) % a.box_n[1], bz = i / (a.box_n[0] * a.box_n[1]);
-            const uint32_t v = (uint32_t)(a.box_o[0] + bx) + (uint32_t)a.nx * (uint32_t)(a.box_o[1] + by) +
-                               nxy * (uint32_t)(boz + bz);
+            const int bx = i % g.n[0], by = (i / g.n[0]) % g.n[1], bz = i / (g.n[0] * g.n[1]);
+            const uint32_t v = (uint32_t)(g.o[0] + bx) + (uint32_t)a.nx * (uint32_t)(g.o[1] + by) +
+                               nxy * (uint32_t)(g.o[2] + bz);
             unsafeAtomicAdd(&a.tally[v], s);
         }
     }
 }
 
-// Placement of the LDS tally box in z.  box_plane_kernel: one wave per z
-// plane sums the deposition field over the box's x-y footprint, skipping
-// voxels removed since (mu_t = 0: ablated after that deposition);
-// box_window_kernel: one lane picks the d consecutive planes with the largest
-// sum (ties: the higher planes; no deposition at all: the top planes).
-__global__ __launch_bounds__(64) void box_plane_kernel(const double* __restrict__ src, const float* __restrict__ mut,
-                                                       int nx, int ny, int ox, int oy, int sx, int sy,
-                                                       double* __restrict__ psum)
+// Placement of the LDS tally box in z.  box_plane_kernel: one block per z
+// plane sums the last run's tally (path length, in proportion to the voxel
+// visits, i.e. the deposits a box would take) over the box's footprint,
+// skipping voxels removed since (mu_t = 0: ablated after that run);
+// box_window_kernel: one block sums every window of d consecutive planes
+// (plane sums staged in LDS, one window per lane) and keeps the largest,
+// ties to the higher window; with no path at all the box sits at the top.
+struct BoxPlace {
+    int sd, d, ox, oy;                             // footprint sd x sd at (ox, oy), d planes
+};
+
+__global__ __launch_bounds__(256) void box_plane_kernel(const double* __restrict__ src, const float* __restrict__ mut,
+                                                        int nx, int ny, BoxPlace bp, double* __restrict__ psum)
 {
+    __shared__ double red[4];
     const int k = blockIdx.x;
     double t = 0.0;
-    for (int q = threadIdx.x; q < sx * sy; q += 64) {
-        const int i = ox + q % sx, j = oy + q / sx;
+    for (int q = threadIdx.x; q < bp.sd * bp.sd; q += 256) {
+        const int i = bp.ox + q % bp.sd, j = bp.oy + q / bp.sd;
         const long v = (long)i + (long)nx * (j + (long)ny * k);
         if (mut[v] > 0.f) t += src[v];
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-    if (threadIdx.x == 0) psum[k] = t;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) psum[k] = red[0] + red[1] + red[2] + red[3];
 }
 
-__global__ __launch_bounds__(64) void box_window_kernel(const double* __restrict__ psum, int nz, int d,
-                                                        int* __restrict__ out)
+constexpr int BOX_MAX_NZ = 4096;                  // tam_create's largest grid edge
+
+__global__ __launch_bounds__(256) void box_window_kernel(const double* __restrict__ psum, int nz, BoxPlace bp,
+                                                         int* __restrict__ out)
 {
-    if (threadIdx.x != 0) return;
-    double w = 0.0;
-    for (int k = nz - d; k < nz; ++k) w += psum[k];
-    double best = w;
-    int at = nz - d;
-    for (int k = nz - d - 1; k >= 0; --k) {       // window [k, k + d)
-        w += psum[k] - psum[k + d];
-        if (w > best) {
+    __shared__ double sp[BOX_MAX_NZ];
+    __shared__ double wb[4];
+    __shared__ int wo[4];
+    for (int k = threadIdx.x; k < nz; k += 256) sp[k] = psum[k];
+    __syncthreads();
+    double best = -1.0;                            // sums are >= 0: any window beats it
+    int bo = -1;
+    for (int o = threadIdx.x; o <= nz - bp.d; o += 256) {   // window [o, o + d)
+        double w = 0.0;
+        for (int k = 0; k < bp.d; ++k) w += sp[o + k];
+        if (w > best || (w == best && o > bo)) {
             best = w;
-            at = k;
+            bo = o;
         }
     }
-    out[0] = at;
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+        const double ow = __shfl_xor(best, s, 64);
+        const int oo = __shfl_xor(bo, s, 64);
+        if (ow > best || (ow == best && oo > bo)) {
+            best = ow;
+            bo = oo;
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wb[threadIdx.x >> 6] = best;
+        wo[threadIdx.x >> 6] = bo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < 4; ++q)
+            if (wb[q] > best || (wb[q] == best && wo[q] > bo)) {
+                best = wb[q];
+                bo = wo[q];
+            }
+        out[0] = bp.ox;
+        out[1] = bp.oy;
+        out[2] = best > 0.0 ? bo : nz - bp.d;
+        out[3] = bp.sd;
+        out[4] = bp.sd;
+        out[5] = bp.d;
+    }
 }
 
 // Many instances in one launch: a wave takes packets from one instance's
@@ -474,7 +523,7 @@ __global__ __launch_bounds__(256) void mc_batch_kernel(const McArgs* __restrict_
         const int i = (wave + k) % n;
         const McArgs a = tab[i];
         if (a.npk == 0) continue;
-        transport<false>(a, nullptr, 0);
+        transport<false>(a, nullptr, BoxGeom{});
     }
 }
 
@@ -528,7 +577,8 @@ hipError_t mc_prepare(int dev)
 
 hipError_t launch_mc(const McArgs& a, hipStream_t s)
 {
-    if (a.npk == 0) return hipSuccess;
+    const long nvox = (long)a.nx * a.ny * a.nz;
+    if (a.npk == 0) return a.clear_tally ? launch_fill_f64(a.tally, nvox, 0.0, s) : hipSuccess;
     int dev = 0, ncu = 0, lds_cu = 0, lds_blk = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -549,40 +599,45 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
     if (need < blocks) blocks = need;
     // LDS tally box: the resident blocks of a CU share its LDS (160 KiB on
     // MI355X, also the most one block may take), 1 KiB per block kept for
-    // the kernel's own use: 27^3 voxels at 1 block per CU, 21^3 at 2, 17^3
-    // at 4.  A cube (s x s x d, d ~ s) on
-    // the beam axis, by default from the top face down, where most of the
-    // path of a surface beam lies; with a deposition field it moves down to
-    // where that field is largest.  On the c1 skin cycles a cube caught more
-    // deposits past the first cycle than flatter (d ~ s/2, s/8) or taller
-    // (d ~ 2s) boxes of the same volume.
+    // the kernel's own use: a cube of 27^3 voxels at 1 block per CU, 21^3 at
+    // 2, 18^2 x 20 at 3, 17^3 at 4, on the beam axis.  Its z range is where
+    // the last run's path was largest, or the top face (where most of the
+    // path of a surface beam lies) before any run.  On the c1 skin cycles a
+    // cube caught more deposits past the first cycle than flatter (d ~ s/2,
+    // s/8) or taller (d ~ 2s) boxes of the same volume, and than a per-launch
+    // choice among depths of 2-32 planes made from the last run's path.
     McArgs b = a;
     b.box_n[0] = b.box_n[1] = b.box_n[2] = 0;
+    b.box_geom = nullptr;
     const long vox = std::min((long)lds_blk - 1024L, (long)lds_cu / bpc - 1024L) / 8L;
-    int sd = (int)std::cbrt((double)vox);
-    sd = std::min(sd, std::min(a.nx, a.ny));
-    const int d = sd > 0 ? std::min<long>(a.nz, vox / ((long)sd * sd)) : 0;
-    if (a.box_enable && sd >= 4 && d >= 1) {
-        const int ix = (int)std::lround(a.cx / a.dx) - sd / 2, iy = (int)std::lround(a.cy / a.dy) - sd / 2;
-        b.box_o[0] = std::max(0, std::min(ix, a.nx - sd));
-        b.box_o[1] = std::max(0, std::min(iy, a.ny - sd));
-        b.box_o[2] = a.nz - d;
-        b.box_n[0] = sd;
-        b.box_n[1] = sd;
-        b.box_n[2] = d;
-        b.box_z = nullptr;
-        if (a.box_src && a.box_psum && a.box_z) {
-            // depth: where the deposition field (the last cycle's) is largest
-            // under the beam in the tissue still present; after ablation that
-            // is below the crater floor
-            hipLaunchKernelGGL(box_plane_kernel, dim3((unsigned)a.nz), dim3(64), 0, s, a.box_src, a.mut, a.nx, a.ny,
-                               b.box_o[0], b.box_o[1], sd, sd, a.box_psum);
-            hipLaunchKernelGGL(box_window_kernel, dim3(1), dim3(64), 0, s, a.box_psum, a.nz, d, a.box_z);
-            b.box_z = a.box_z;
+    BoxPlace bp{};
+    bp.sd = std::min((int)std::cbrt((double)vox), std::min(a.nx, a.ny));
+    bp.d = bp.sd > 0 ? (int)std::min<long>(a.nz, vox / ((long)bp.sd * bp.sd)) : 0;
+    if (a.box_enable && bp.sd >= 4 && bp.d >= 1) {
+        bp.ox = std::max(0, std::min((int)std::lround(a.cx / a.dx) - bp.sd / 2, a.nx - bp.sd));
+        bp.oy = std::max(0, std::min((int)std::lround(a.cy / a.dy) - bp.sd / 2, a.ny - bp.sd));
+        b.box_o[0] = bp.ox;
+        b.box_o[1] = bp.oy;
+        b.box_o[2] = a.nz - bp.d;
+        b.box_n[0] = b.box_n[1] = bp.sd;
+        b.box_n[2] = bp.d;
+        const size_t lds = sizeof(double) * (size_t)bp.sd * (size_t)bp.sd * (size_t)bp.d;
+        if (a.box_psum && a.box_geom) {
+            hipLaunchKernelGGL(box_plane_kernel, dim3((unsigned)a.nz), dim3(256), 0, s, a.tally, a.mut, a.nx, a.ny,
+                               bp, a.box_psum);
+            hipLaunchKernelGGL(box_window_kernel, dim3(1), dim3(256), 0, s, a.box_psum, a.nz, bp, a.box_geom);
+            b.box_geom = a.box_geom;
         }
-        const size_t lds = sizeof(double) * (size_t)sd * (size_t)sd * (size_t)d;
+        if (a.clear_tally) {
+            const hipError_t ef = launch_fill_f64(a.tally, nvox, 0.0, s);
+            if (ef != hipSuccess) return ef;
+        }
         hipLaunchKernelGGL(mc_box_kernel, dim3((unsigned)blocks), dim3(256), lds, s, b);
     } else {
+        if (a.clear_tally) {
+            const hipError_t ef = launch_fill_f64(a.tally, nvox, 0.0, s);
+            if (ef != hipSuccess) return ef;
+        }
         hipLaunchKernelGGL(mc_kernel, dim3((unsigned)blocks), dim3(256), 0, s, b);
     }
     return hipGetLastError();

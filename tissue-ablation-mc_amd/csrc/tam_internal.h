@@ -79,13 +79,14 @@ struct McArgs {
     // block-private FP64 tally box in LDS (box_n[0] == 0: none): voxels
     // [box_o, box_o + box_n) are summed per block and added to the tally once
     int box_o[3], box_n[3];
-    // box depth from a deposition field: launch_mc places the box's z range
-    // where box_src (interior, e.g. the last cycle's jmean) sums highest
-    // inside the box's x-y footprint, through box_psum (nz doubles) into
-    // *box_z, which the box kernel reads (nullptr: box_o[2] as given)
-    const double* box_src;
+    // box depth from the last run's path: before clearing the tally,
+    // launch_mc picks the z range whose planes held the most of the tally's
+    // path length inside the box's footprint (over voxels still with
+    // mu_t > 0), through box_psum (nz doubles) into box_geom (o[3], n[3]),
+    // which the box kernel reads (nullptr: box_o / box_n as given)
     double* box_psum;
-    int* box_z;
+    int* box_geom;
+    int clear_tally;           // launch_mc zeroes the tally (after the placement) before the transport
 };
 
 // counters[TAM_CNT_N] is the transport's packet queue head
@@ -240,8 +241,8 @@ struct tam_ctx {
     double* mua_frac;           // per material mu_a / mu_t (FP64)
     double *tally, *jmean, *Q, *W, *omega, *thres;
     uint8_t* tmask;             // bit k: thres[k] != 0 (kept with every write of thres)
-    double* box_psum;           // LDS tally box placement: per-plane sums (nz) and the chosen z origin
-    int* box_z;
+    double* box_psum;           // LDS tally box placement: per-plane sums (nz) and the chosen
+    int* box_geom;              // geometry (o[3], n[3])
     double* energy;             // [0] boundary outflow, [1..] reduction scratch
     int* flags;
     unsigned long long* counters;
