@@ -584,7 +584,8 @@ void tam_destroy(tam_ctx* c)
                     c->heatcap, c->coeff, c->rhokap, c->mut, c->mat, c->mask, c->mtab, c->mua_frac,
                     c->tally, c->jmean, c->Q, c->W, c->omega, c->thres, c->tmask, c->energy, c->flags,
                     c->counters, c->T64a, c->T64b, c->ax, c->ay, c->az, c->cls, c->glut,
-                    c->robin_lut, c->eblk, c->box_psum, c->box_geom};
+                    c->robin_lut, c->eblk, c->box_psum, c->box_geom, c->tn_omega, c->tn_thres, c->tn_Q,
+                    c->tn_eblk, c->tn_tmask};
     for (void* q : ptrs)
         if (q) hipFree(q);
     cycle_reset(c);
@@ -1240,13 +1241,15 @@ int tam_tune_heat(tam_ctx* c, int32_t reps, int32_t* tile, int32_t* kz, double* 
         // the work it does in the run.
         HeatArgs a = heat_args(c);
         const long V = c->nvox;
-        double *om = nullptr, *th = nullptr, *eb = nullptr, *qs = nullptr;
-        uint8_t* tm = nullptr;
-        HIPCHK(c, hipMalloc((void**)&om, sizeof(double) * (size_t)V));
-        hipError_t e = hipMalloc((void**)&th, sizeof(double) * 3 * (size_t)V);
-        if (e == hipSuccess) e = hipMalloc((void**)&eb, sizeof(double) * (size_t)c->neblk);
-        if (e == hipSuccess) e = hipMalloc((void**)&qs, sizeof(double) * (size_t)V);
-        if (e == hipSuccess) e = hipMalloc((void**)&tm, (size_t)V);
+        // scratch: allocated once per context (the tuner runs every few cycles)
+        hipError_t e = hipSuccess;
+        if (!c->tn_omega) e = dalloc(&c->tn_omega, V);
+        if (e == hipSuccess && !c->tn_thres) e = dalloc(&c->tn_thres, 3 * V);
+        if (e == hipSuccess && !c->tn_eblk) e = dalloc(&c->tn_eblk, c->neblk);
+        if (e == hipSuccess && !c->tn_Q) e = dalloc(&c->tn_Q, V);
+        if (e == hipSuccess && !c->tn_tmask) e = dalloc(&c->tn_tmask, V);
+        double *om = c->tn_omega, *th = c->tn_thres, *eb = c->tn_eblk, *qs = c->tn_Q;
+        uint8_t* tm = c->tn_tmask;
         if (e == hipSuccess) e = hipMemcpyAsync(om, c->omega, sizeof(double) * (size_t)V, hipMemcpyDeviceToDevice,
                                                 c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(qs, c->Q, sizeof(double) * (size_t)V, hipMemcpyDeviceToDevice,
@@ -1290,11 +1293,6 @@ int tam_tune_heat(tam_ctx* c, int32_t reps, int32_t* tile, int32_t* kz, double* 
             }
         }
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        (void)hipFree(om);
-        (void)hipFree(th);
-        (void)hipFree(eb);
-        (void)hipFree(qs);
-        (void)hipFree(tm);
         if (e != hipSuccess) return hip_fail(c, "heat tuning", e);
         if (bt != c->tile || bk != c->kz) cycle_reset(c);     // captured cycles use the old shape
         c->tile = bt;

@@ -241,6 +241,9 @@ struct tam_ctx {
     double* mua_frac;           // per material mu_a / mu_t (FP64)
     double *tally, *jmean, *Q, *W, *omega, *thres;
     uint8_t* tmask;             // bit k: thres[k] != 0 (kept with every write of thres)
+    // heat tuner scratch copies (allocated on the first tam_tune_heat, kept)
+    double *tn_omega, *tn_thres, *tn_Q, *tn_eblk;
+    uint8_t* tn_tmask;
     double* box_psum;           // LDS tally box placement: per-plane sums (nz) and the chosen
     int* box_geom;              // geometry (o[3], n[3])
     double* energy;             // [0] boundary outflow, [1..] reduction scratch

@@ -40,15 +40,20 @@ class CycleRunner:
             self.totals[k] = 0.0
         self.heat_by_shape = {}
 
+    def _retune(self):
+        """The damage window and the source change from cycle to cycle, fastest
+        early on: re-time the stencil shapes on the field of cycles 0, 1, 2, 4,
+        8, 16, ... (long cycles only)."""
+        c = self.cycle
+        return self.tuned is not None and self.prob.steps >= 200 and (c & (c - 1)) == 0
+
     def run_cycle(self):
         if self.graph:
             return self._run_cycle_graph()
         p, g = self.prob, self.grid
         st = g.mc_run(p.packets, seed=self.seed, stream=self.cycle)
         g.normalise(p.scale)
-        if self.tuned is not None and p.steps >= 200:
-            # the damage window and the source change from cycle to cycle:
-            # re-time the stencil shapes on this cycle's field
+        if self._retune():
             self.tuned = g.tune_heat(reps=3)
         g.heat_run(p.steps)
         self.time += p.steps * p.params.delt
@@ -77,7 +82,7 @@ class CycleRunner:
 
     def _run_cycle_graph(self):
         p, g = self.prob, self.grid
-        if self.tuned is not None and p.steps >= 200:
+        if self._retune():
             self.tuned = g.tune_heat(reps=3)         # keeps the graph unless the shape changes
         arr = 0.0 if p.params.damage_every_step else p.params.delt
         st = g.cycle(p.packets, self.seed, self.cycle, p.scale, p.steps, arr_delt=arr)
