@@ -17,11 +17,13 @@
 //     voxel's +x/+y/+z face; the -faces are the neighbours' +faces.
 // Both forms hold bit-identical G values (same FP64 formula, then rounded).
 //
-// Kernel shape (2.5D blocking): a TX x TY block owns an x-y tile and marches
-// KZ planes in z.  Each plane of dT (and classes or gx/gy) is staged in LDS
-// with a one-cell halo so the +-x/+-y neighbours come from LDS; the +-z
-// neighbours stay in registers.  Loads for plane k+1 are issued before plane
-// k is computed, so they are in flight during the LDS exchange and the math.
+// Kernel shapes (2.5D blocking): a TX x TY block owns an x-y tile and marches
+// KZ planes in z.  stencil_kernel stages each plane of dT (and classes or
+// gx/gy) in LDS with a one-cell halo so the +-x/+-y neighbours come from LDS;
+// stencil_direct_kernel loads them directly (cache hits); the +-z neighbours
+// stay in registers.  stencil_t2_kernel runs two steps per launch (temporal
+// blocking, class mode): the first step on a footprint one cell wider than
+// the owned tile, handed to the second through LDS.
 #include "tam_internal.h"
 
 namespace {

@@ -34,7 +34,11 @@
 //    absorption).  The absorbed fraction per voxel is mu_a * L.  Voxels with
 //    mu_t = 0 absorb nothing and are crossed without a tally.  The global
 //    atomic rate (~2.4e10/s on MI355X whatever the address spread) is the
-//    bound of this kernel: without the tally it runs ~4.5x faster.
+//    bound of this kernel: without the tally it runs ~4.5x faster.  Each
+//    block therefore sums the visits to a cube of voxels on the beam axis in
+//    LDS (mc_box_kernel), placed in z where the last run's path was largest
+//    (box_plane_kernel, box_window_kernel), and adds the cube to the tally
+//    once at exit.
 //  * Transcendentals use the hardware approximations (v_log, v_sin/v_cos
 //    in revolutions, v_sqrt, v_rcp/v_rsq; ~1 ulp): they only shape sampled
 //    distances and directions, whose distributions the parity tests check.
