@@ -180,14 +180,19 @@ int tam_mc_run(tam_ctx* ctx, int64_t npackets, uint64_t seed, uint32_t stream, i
  * and off.  None of them changes results beyond FP64 summation order:
  * packet p always draws from its own stream.
  * tam_set_mc_box: each block sums the path it lays in a cube of voxels on
- * the beam axis (at most 64 KiB of FP64, sized to the blocks per CU) in LDS
- * and adds the box to the tally once at the end, instead of one device
- * atomic per voxel visit (1 = on, the default).  The cube's z range is
- * where the current jmean (the last normalised deposition), over voxels
- * with mu_t > 0, sums highest inside its x-y footprint; with jmean zero it
- * starts at the top face. */
+ * the beam axis in LDS (FP64; the CU's LDS shared by the blocks per CU, at
+ * most the device's per-block limit) and adds the box to the tally once at
+ * the end, instead of one device atomic per voxel visit (1 = on, the
+ * default).  Each transport launch places the cube's z range where the
+ * tally it is about to clear (the last run's path), over voxels with
+ * mu_t > 0, sums highest inside the cube's x-y footprint (ties: the higher
+ * range); with no path yet the cube starts at the top face.
+ * tam_get_mc_box: the box of the last launch that placed one (no reference
+ * counterpart; for tests and tuning): geom = {x0, y0, z0, nx, ny, nz}, 0-based
+ * interior voxel indices; waits for the context's stream. */
 int tam_set_mc_launch(tam_ctx* ctx, int32_t blocks_per_cu, int32_t rng_batch);
 int tam_set_mc_box(tam_ctx* ctx, int32_t enable);
+int tam_get_mc_box(tam_ctx* ctx, int32_t* geom);
 int tam_tune_mc(tam_ctx* ctx, int64_t sample, int32_t* blocks_per_cu, int32_t* rng_batch,
                 double* packets_per_s);
 

@@ -173,6 +173,43 @@ def test_mc_box_follows_the_deposition_below_a_crater():
     assert np.abs(t0 - t_or).sum() / t_or.sum() < 1e-2
 
 
+def test_mc_box_placement_is_the_best_window():
+    """The box's z range is the window of planes whose path (the tally the
+    launch clears, over voxels with mu_t > 0, inside the box's footprint)
+    is largest: recomputed in numpy from the downloaded tally and mu_t.
+    Also: the footprint is centred on the beam, and a fresh grid (no path
+    yet) gets the top planes."""
+    prob = _two_layer_problem(n=40, packets=200_000)
+    n = 40
+    mua = np.array([t[0] + t[1] for t in prob.table])
+    rk = np.zeros((n + 2,) * 3)
+    rk[1:-1, 1:-1, 1:-1] = mua[prob.mat]
+    zz, yy, xx = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    crater = ((xx - n / 2 + 0.5) ** 2 + (yy - n / 2 + 0.5) ** 2 <= 30) & (zz >= n - 15)
+    rk[1:-1, 1:-1, 1:-1][crater] = 0.0
+    prob.rhokap = rk
+    g = prob.grid()
+    g.set_mc_launch(4, 32)
+    g.set_mc_box(1)
+    g.mc_run(prob.packets, seed=5, stream=0)
+    (x0, y0, z0), (sx, sy, sz) = g.mc_box()
+    _print("fresh grid: box", (x0, y0, z0), (sx, sy, sz))
+    assert sx == sy and z0 == n - sz                           # top planes without a path
+    assert abs((x0 + sx / 2) - n / 2) <= 1 and abs((y0 + sy / 2) - n / 2) <= 1
+    tally = g.download(lib.FIELD_TALLY)                        # the path the next launch places from
+    mut = g.download(lib.FIELD_RHOKAP)[1:-1, 1:-1, 1:-1]
+    g.mc_run(prob.packets, seed=6, stream=0)
+    (x1, y1, z1), n1 = g.mc_box()
+    g.close()
+    assert (x1, y1) == (x0, y0) and n1 == (sx, sy, sz)
+    foot = np.where(mut > 0, tally, 0.0)[:, y0:y0 + sy, x0:x0 + sx].sum(axis=(1, 2))
+    win = np.array([foot[o:o + sz].sum() for o in range(n - sz + 1)])
+    best = win.max()
+    _print("placed at z", z1, "window sum", win[z1], "best", best, "at", int(np.flatnonzero(win == best)[-1]))
+    assert z1 < n - sz                                         # moved below the crater floor
+    assert win[z1] >= best * (1 - 1e-12)                       # the best window (to FP64 summation order)
+
+
 def _batches(g, nb, per, seed):
     out = []
     for b in range(nb):

@@ -523,6 +523,7 @@ int tam_create(const tam_params* p, tam_ctx** out)
     if (e == hipSuccess) e = dalloc(&c->tmask, V);
     if (e == hipSuccess) e = dalloc(&c->box_psum, nz);
     if (e == hipSuccess) e = dalloc(&c->box_geom, 6);
+    if (e == hipSuccess) e = hipMemsetAsync(c->box_geom, 0, 6 * sizeof(int), c->stream);
     if (e == hipSuccess) e = dalloc(&c->energy, 4);
     if (e == hipSuccess) e = dalloc(&c->flags, 4);
     if (e == hipSuccess) e = dalloc(&c->counters, TAM_CNT_SLOTS);
@@ -1176,6 +1177,17 @@ int tam_set_mc_box(tam_ctx* c, int32_t enable)
     if (!c) return TAM_ERR_ARG;
     cycle_reset(c);
     c->mc_box = enable != 0;
+    return TAM_OK;
+}
+
+int tam_get_mc_box(tam_ctx* c, int32_t* geom)
+{
+    if (!c || !geom) return TAM_ERR_ARG;
+    Dev d(c->dev);
+    int g[6];
+    HIPCHK(c, hipMemcpyAsync(g, c->box_geom, sizeof(g), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < 6; ++i) geom[i] = g[i];
     return TAM_OK;
 }
 

@@ -27,7 +27,7 @@ EXPORTED = ("tam_abi_version", "tam_create", "tam_destroy", "tam_last_error", "t
             "tam_set_thermal", "tam_mc_run", "tam_normalise", "tam_heat_run", "tam_arrhenius",
             "tam_rebuild", "tam_download", "tam_upload", "tam_energy", "tam_get_timing",
             "tam_synchronize", "tam_set_heat_launch", "tam_tune_heat", "tam_set_mc_launch", "tam_tune_mc",
-            "tam_set_mc_box",
+            "tam_set_mc_box", "tam_get_mc_box",
             "tam_copy_bandwidth", "tam_atomic_rate", "tam_struct_size", "tam_pulse_advance", "tam_cycle",
             "tam_tally_blocks", "tam_batch_create", "tam_batch_destroy", "tam_batch_last_error",
             "tam_batch_mc_run", "tam_batch_heat_run", "tam_batch_set_launch", "tam_batch_tune", "tam_batch_last_ms")
@@ -127,6 +127,7 @@ def load():
     L.tam_set_heat_launch.argtypes = [vp, C.c_int32, C.c_int32]
     L.tam_set_mc_launch.argtypes = [vp, C.c_int32, C.c_int32]
     L.tam_set_mc_box.argtypes = [vp, C.c_int32]
+    L.tam_get_mc_box.argtypes = [vp, C.POINTER(C.c_int32)]
     L.tam_copy_bandwidth.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
     L.tam_atomic_rate.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(C.c_double)]
     L.tam_tally_blocks.argtypes = [vp, C.c_int32, dp]
@@ -298,6 +299,13 @@ class TissueGrid:
     def set_mc_box(self, enable):
         """Block-private LDS tally box on the beam axis on (default) or off."""
         self._chk(self.L.tam_set_mc_box(self.h, 1 if enable else 0))
+
+    def mc_box(self):
+        """The LDS box of the last transport launch that placed one:
+        ((x0, y0, z0), (nx, ny, nz)), 0-based interior voxel indices."""
+        g = (C.c_int32 * 6)()
+        self._chk(self.L.tam_get_mc_box(self.h, g))
+        return tuple(g[:3]), tuple(g[3:])
 
     def tune_mc(self, sample=2_000_000):
         """Autotune the transport launch; returns (blocks per CU, random-phase
