@@ -519,15 +519,59 @@ __global__ __launch_bounds__(256) void box_window_kernel(const double* __restric
 // queue until it drains, then moves on to the next instance (starting at its
 // own index, so the waves spread over the instances); each instance keeps its
 // own queue, tally and counters, and the instance's arguments stay uniform
-// over the wave.
-__global__ __launch_bounds__(256) void mc_batch_kernel(const McArgs* __restrict__ tab, int n)
+// over the wave.  With wbox > 0 each wave owns an LDS slice of wbox FP64
+// voxels: for every instance it works on (with the box enabled) it sums its
+// visits to a cube under the top face on that instance's beam axis there,
+// and adds the cube to the instance's tally when it moves on.  The waves of
+// a block work on different instances, so the slices are per wave and only
+// wave-level ordering is needed.
+__global__ __launch_bounds__(256) void mc_batch_kernel(const McArgs* __restrict__ tab, int n, int wbox)
 {
+    extern __shared__ double lds_slices[];
+    double* box = lds_slices + (size_t)(threadIdx.x >> 6) * (size_t)wbox;
+    const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     for (int k = 0; k < n; ++k) {
         const int i = (wave + k) % n;
         const McArgs a = tab[i];
         if (a.npk == 0) continue;
-        transport<false>(a, nullptr, BoxGeom{});
+        BoxGeom g{};
+        if (wbox > 0 && a.box_enable) {
+            // a cube of at most wbox voxels inside the grid (sd <= nx, ny; d <= nz)
+            int sd = (int)cbrtf((float)wbox);
+            while ((long)(sd + 1) * (sd + 1) * (sd + 1) <= wbox) ++sd;
+            while (sd > 1 && (long)sd * sd * sd > wbox) --sd;
+            sd = min(sd, min(a.nx, a.ny));
+            const int d = min(a.nz, wbox / (sd * sd));
+            g.n[0] = g.n[1] = sd;
+            g.n[2] = d;
+            g.o[0] = max(0, min((int)lroundf(a.cx / a.dx) - sd / 2, a.nx - sd));
+            g.o[1] = max(0, min((int)lroundf(a.cy / a.dy) - sd / 2, a.ny - sd));
+            g.o[2] = a.nz - d;
+        }
+        const int nb = g.n[0] * g.n[1] * g.n[2];
+        if (nb < 8) {
+            transport<false>(a, nullptr, g);
+            continue;
+        }
+        for (int q = lane; q < nb; q += 64) box[q] = 0.0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");   // the zeros land before any add
+        __builtin_amdgcn_wave_barrier();
+        transport<true>(a, box, g);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");   // every lane's adds land before the flush
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t nxy = (uint32_t)a.nx * (uint32_t)a.ny;
+        for (int q = lane; q < nb; q += 64) {
+            const double s = box[q];
+            if (s != 0.0) {
+                const int bx = q % g.n[0], by = (q / g.n[0]) % g.n[1], bz = q / (g.n[0] * g.n[1]);
+                const uint32_t v = (uint32_t)(g.o[0] + bx) + (uint32_t)a.nx * (uint32_t)(g.o[1] + by) +
+                                   nxy * (uint32_t)(g.o[2] + bz);
+                unsafeAtomicAdd(&a.tally[v], s);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");   // the flush reads finish before the next zeros
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -563,10 +607,23 @@ hipError_t launch_mc_batch(const McArgs* dtab, int n, long total_npk, int blocks
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess || ncu <= 0) ncu = 256;
-    long blocks = (long)ncu * (blocks_per_cu > 0 ? blocks_per_cu : 8);
+    const int bpc = blocks_per_cu > 0 ? blocks_per_cu : 8;
+    long blocks = (long)ncu * bpc;
     const long need = (total_npk + 255) / 256;
     if (need < blocks) blocks = need;
-    hipLaunchKernelGGL(mc_batch_kernel, dim3((unsigned)blocks), dim3(256), 0, s, dtab, n);
+    // per-wave LDS tally boxes: the block's share of the CU's LDS (1 KiB kept
+    // for the kernel's own use), split over its 4 waves
+    int lds_cu = 0, lds_blk = 0;
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+        lds_cu <= 0)
+        lds_cu = 64 * 1024;
+    if (hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        lds_blk <= 0)
+        lds_blk = 64 * 1024;
+    const long per_block = std::min((long)lds_blk - 1024L, (long)lds_cu / bpc - 1024L);
+    const int wbox = per_block > 0 ? (int)(per_block / 4 / 8) : 0;
+    const size_t lds = sizeof(double) * 4 * (size_t)wbox;
+    hipLaunchKernelGGL(mc_batch_kernel, dim3((unsigned)blocks), dim3(256), lds, s, dtab, n, wbox);
     return hipGetLastError();
 }
 
@@ -575,8 +632,10 @@ hipError_t mc_prepare(int dev)
     int lds_blk = 0;
     hipError_t e = hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (e != hipSuccess || lds_blk <= 64 * 1024) return e;
-    // allow the box kernel the device's whole per-block LDS (outside any graph capture)
-    return hipFuncSetAttribute((const void*)mc_box_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_blk);
+    // allow the box kernels the device's whole per-block LDS (outside any graph capture)
+    e = hipFuncSetAttribute((const void*)mc_box_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_blk);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void*)mc_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_blk);
 }
 
 hipError_t launch_mc(const McArgs& a, hipStream_t s)
