@@ -161,6 +161,99 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs a)
     }
 }
 
+// Convolutions with at most 32 output channels, their input staged in LDS.
+// In conv_kernel each A fragment of these layers feeds one or two MFMAs, so
+// the vector-memory path, not the matrix cores, sets the rate.  Here a block
+// computes an output brick of 16 (x) x 8 (y) x 4 (z) voxels.  For each group
+// of 16 input channels it loads the input brick with a one-voxel halo
+// (18 x 10 x 6 voxels x 16 channels, zeros outside the grid) into LDS, and
+// every tap's A fragment is then an LDS read at a constant offset, with no
+// per-tap validity test.  Wave w computes the 8 x-rows of plane z0 + w.  The
+// k loop runs channel group by channel group (27 taps x 16 channels each, two
+// taps per k-block), so each lane fetches its B fragment from the packed
+// weights at that (tap, channel)'s place in their tap-major order.
+// Grid edges must be multiples of 16 (x), 8 (y) and 4 (z); CIN a multiple of
+// 16; NT = output channels / 16.
+template <int CIN, int NT>
+__global__ __launch_bounds__(256) void conv_lds_kernel(ConvArgs a)
+{
+    constexpr int BX = 18, BY = 10, BZ = 6, CG = CIN / 16;
+    constexpr int KB = (27 * CIN + 31) / 32;                  // k-blocks of the packed weights
+    __shared__ __attribute__((aligned(16))) uint16_t sx[BZ * BY * BX * 16];
+    const int W = a.W, H = a.H, D = a.D;
+    const int gx = W / 16, gy = H / 8;
+    const int bxi = blockIdx.x % gx, byi = (blockIdx.x / gx) % gy, bzi = blockIdx.x / (gx * gy);
+    const int x0 = bxi * 16, y0 = byi * 8, z0 = bzi * 4;
+    const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15, w = threadIdx.x >> 6;
+    const Rsrc rw = rsrc_n(a.wp, (long)KB * NT * 512 * 2);
+    f32x4 acc[8][NT];
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[t][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int cg = 0; cg < CG; ++cg) {
+        if (cg > 0) __syncthreads();                          // the previous group's reads are done
+        // stage channels 16 cg .. 16 cg + 15 of the input brick: 16-byte chunks, two per voxel
+        for (int q = threadIdx.x; q < BZ * BY * BX * 2; q += 256) {
+            const int vq = q >> 1, half = q & 1;
+            const int lx = vq % BX, ly = (vq / BX) % BY, lz = vq / (BX * BY);
+            const int xx = x0 - 1 + lx, yy = y0 - 1 + ly, zz = z0 - 1 + lz;
+            bf16x8 v = {};
+            if (xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D)
+                v = *(const bf16x8*)(a.x + (((long)zz * H + yy) * W + xx) * CIN + 16 * cg + half * 8);
+            *(bf16x8*)(sx + vq * 16 + half * 8) = v;
+        }
+        __syncthreads();
+        // not unrolled: with the tap offsets as constants the compiler keeps all
+        // 14 k-blocks' addresses live across the channel-group loop
+#pragma unroll 1
+        for (int kb = 0; kb < 14; ++kb) {
+            // lane group g: tap 2 kb + g / 2, channels 16 cg + 8 (g % 2) .. + 7
+            const int tap = 2 * kb + (g >> 1), ci0 = 8 * (g & 1);
+            bf16x8 af[8], bfr[NT];
+            if (tap < 27) {
+                const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+                const int base = ((w + dz + 1) * BY + (dy + 1)) * BX + (r + dx + 1);
+#pragma unroll
+                for (int t = 0; t < 8; ++t) af[t] = *(const bf16x8*)(sx + (base + t * BX) * 16 + ci0);
+                // this lane's k in the packed (tap-major) order: its k-block and lane group there
+                const int ko = tap * CIN + 16 * cg + ci0;
+                const int kbo = ko >> 5, go = (ko >> 3) & 3;
+#pragma unroll
+                for (int n = 0; n < NT; ++n)
+                    bfr[n] = bload8(rw, (unsigned)((go * 16 + r) * 16), (unsigned)((kbo * NT + n) * 1024));
+            } else {
+#pragma unroll
+                for (int t = 0; t < 8; ++t) af[t] = bf16x8{};
+#pragma unroll
+                for (int n = 0; n < NT; ++n) bfr[n] = bf16x8{};
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+#pragma unroll
+                for (int n = 0; n < NT; ++n)
+                    acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[t], bfr[n], acc[t][n], 0, 0, 0);
+        }
+    }
+    // lane holds voxels x0 + 4 (lane / 16) + j of row (y0 + t, z0 + w), channel 16 n + lane % 16
+    constexpr int COUT = NT * 16;
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+        const int co = n * 16 + (lane & 15);
+        const float bb = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long m = ((long)(z0 + w) * H + (y0 + t)) * W + x0 + (lane >> 4) * 4 + j;
+                float v = acc[t][n][j] + bb;
+                if (a.relu) v = fmaxf(v, 0.f);
+                if (a.mask && !(bf2f(a.mask[m * COUT + co]) > 0.f)) v = 0.f;
+                a.y[m * COUT + co] = f2bf(v);
+            }
+    }
+}
+
 struct WgradArgs {
     const uint16_t* x;      // layer input [nvox][cin]
     const uint16_t* dy;     // output gradient [nvox][cout]
@@ -627,6 +720,21 @@ hipError_t conv_launch(tam_dn* n, const uint16_t* x, int lvl, int cin, const uin
     const unsigned b8 = (unsigned)(nv / (4 * 16 * 8)), b4 = (unsigned)(nv / (4 * 16 * 4));
 #define CV(CI, NT_, MT_) \
     hipLaunchKernelGGL((conv_kernel<CI, NT_, MT_>), dim3(MT_ == 8 ? b8 : b4), dim3(256), 0, n->stream, a)
+    const int D = n->D[lvl], H = n->H[lvl], W = n->W[lvl];
+    if (W % 16 == 0 && H % 8 == 0 && D % 4 == 0) {
+        // LDS-staged input for the layers with 16 or 32 output channels
+        const dim3 grid((unsigned)((W / 16) * (H / 8) * (D / 4)));
+#define CL(CI, NT_) hipLaunchKernelGGL((conv_lds_kernel<CI, NT_>), grid, dim3(256), 0, n->stream, a)
+        switch (cin * 1000 + cout) {
+        case 16016: CL(16, 1); return hipGetLastError();
+        case 16032: CL(16, 2); return hipGetLastError();
+        case 32032: CL(32, 2); return hipGetLastError();
+        case 96032: CL(96, 2); return hipGetLastError();
+        case 48016: CL(48, 1); return hipGetLastError();
+        case 32016: CL(32, 1); return hipGetLastError();
+        }
+#undef CL
+    }
     switch (cin * 1000 + cout) {
     case 16016: CV(16, 1, 4); break;      // layers 0, 1, 9 and the data gradients of 0, 1, 9
     case 16032: CV(16, 2, 4); break;      // layer 2
