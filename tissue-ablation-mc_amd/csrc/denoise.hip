@@ -351,22 +351,137 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a)
         if (sR[q] != 0.f) atomicAdd(&a.gw[(long)tap * CI * CO + q], sR[q]);
 }
 
-// bias gradient: gb[c] += sum_m d[m][c]
+// Weight gradient of the three taps dx = -1, 0, 1 that share (dz, dy) = blockIdx.y over a range of 32-voxel row
+// segments (rows of W % 32 == 0 voxels, so a segment never leaves its row).  Each wave stages its segment's dy and
+// the 34 x voxels x0-1 .. x0+32 of the source row (z+dz, y+dy) in its own LDS slice, channel-major; the operand
+// of tap dx is the aligned 8-voxel fragment shifted by one bf16 (v_alignbyte on dword pairs), so one staging
+// serves three taps.  Segments whose source row lies outside the grid contribute nothing and are skipped.
+template <int CIT, int COT>
+__global__ __launch_bounds__(256) void wgrad3_kernel(WgradArgs a)
+{
+    constexpr int CI = CIT * 16, CO = COT * 16;
+    constexpr int XR = 48;                                  // x row: element 8 + v holds voxel x0 + v (v = -1 .. 32)
+    constexpr int DR = 40;                                  // dy row: 32 voxels + pad against bank conflicts
+    __shared__ __attribute__((aligned(16))) uint16_t sX[4][CI][XR];
+    __shared__ __attribute__((aligned(16))) uint16_t sD[4][CO][DR];
+    __shared__ float sR[CI * CO];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int dz = (int)blockIdx.y / 3 - 1, dy = (int)blockIdx.y % 3 - 1;
+    f32x4 acc[3][COT][CIT];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int i = 0; i < COT; ++i)
+#pragma unroll
+            for (int c = 0; c < CIT; ++c) acc[s][i][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const long b0 = (long)blockIdx.x * a.per_wg;
+    const long b1 = b0 + a.per_wg < a.blocks ? b0 + a.per_wg : a.blocks;
+    const int k8 = (lane >> 4) * 8, r16 = lane & 15;
+    for (long blk = b0 + wv; blk < b1; blk += 4) {
+        const long m0 = blk * 32;
+        const int x0 = (int)(m0 % a.W);
+        const long r = m0 / a.W;
+        const int sy = (int)(r % a.H) + dy, sz = (int)(r / a.H) + dz;
+        if (sy < 0 || sy >= a.H || sz < 0 || sz >= a.D) continue;      // wave-uniform
+        for (int q = lane; q < 32 * CO / 8; q += 64) {
+            const int v = q / (CO / 8), c8 = (q % (CO / 8)) * 8;
+            const uint4 d = *(const uint4*)(a.dy + (m0 + v) * a.cout + c8);
+            const uint16_t* e = (const uint16_t*)&d;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sD[wv][c8 + j][v] = e[j];
+        }
+        const long srow = ((long)sz * a.H + sy) * a.W;
+        for (int q = lane; q < 34 * CI / 8; q += 64) {
+            const int v = q / (CI / 8) - 1, c8 = (q % (CI / 8)) * 8;
+            const int sx = x0 + v;
+            uint4 d = make_uint4(0u, 0u, 0u, 0u);
+            if (sx >= 0 && sx < a.W) d = *(const uint4*)(a.x + (srow + sx) * a.cin + c8);
+            const uint16_t* e = (const uint16_t*)&d;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sX[wv][c8 + j][8 + v] = e[j];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");         // the slice is written before it is read
+        __builtin_amdgcn_wave_barrier();
+        bf16x8 bx[3][CIT];
+#pragma unroll
+        for (int c = 0; c < CIT; ++c) {
+            const uint16_t* row = &sX[wv][c * 16 + r16][0];
+            const uint4 mid = *(const uint4*)(row + 8 + k8);                  // voxels k8 .. k8+7
+            const unsigned prev = *(const unsigned*)(row + 6 + k8);          // high half: voxel k8-1
+            const unsigned next = *(const unsigned*)(row + 16 + k8);         // low half: voxel k8+8
+            uint4 lo, hi;
+            lo.x = __builtin_amdgcn_alignbyte(mid.x, prev, 2);
+            lo.y = __builtin_amdgcn_alignbyte(mid.y, mid.x, 2);
+            lo.z = __builtin_amdgcn_alignbyte(mid.z, mid.y, 2);
+            lo.w = __builtin_amdgcn_alignbyte(mid.w, mid.z, 2);
+            hi.x = __builtin_amdgcn_alignbyte(mid.y, mid.x, 2);
+            hi.y = __builtin_amdgcn_alignbyte(mid.z, mid.y, 2);
+            hi.z = __builtin_amdgcn_alignbyte(mid.w, mid.z, 2);
+            hi.w = __builtin_amdgcn_alignbyte(next, mid.w, 2);
+            bx[0][c] = __builtin_bit_cast(bf16x8, lo);
+            bx[1][c] = __builtin_bit_cast(bf16x8, mid);
+            bx[2][c] = __builtin_bit_cast(bf16x8, hi);
+        }
+#pragma unroll
+        for (int i = 0; i < COT; ++i) {
+            const bf16x8 ad = load8(&sD[wv][i * 16 + r16][k8]);
+#pragma unroll
+            for (int s = 0; s < 3; ++s)
+#pragma unroll
+                for (int c = 0; c < CIT; ++c)
+                    acc[s][i][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ad, bx[s][c], acc[s][i][c], 0, 0, 0);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");         // the reads finish before the next writes
+        __builtin_amdgcn_wave_barrier();
+    }
+    // per tap: reduce the four waves in LDS, then one atomic per weight
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        for (int q = threadIdx.x; q < CI * CO; q += 256) sR[q] = 0.f;
+        __syncthreads();
+        for (int w = 0; w < 4; ++w) {
+            if (wv == w) {
+#pragma unroll
+                for (int i = 0; i < COT; ++i)
+#pragma unroll
+                    for (int c = 0; c < CIT; ++c)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int co = i * 16 + (lane >> 4) * 4 + j, ci = c * 16 + (lane & 15);
+                            sR[ci * CO + co] += acc[s][i][c][j];
+                        }
+            }
+            __syncthreads();
+        }
+        const int tap = (int)blockIdx.y * 3 + s;
+        for (int q = threadIdx.x; q < CI * CO; q += 256)
+            if (sR[q] != 0.f) atomicAdd(&a.gw[(long)tap * CI * CO + q], sR[q]);
+        __syncthreads();
+    }
+}
+
+// bias gradient: gb[c] += sum_m d[m][c], c a multiple of 8 (<= 2048); each lane reads 8 channels of a voxel
 __global__ __launch_bounds__(256) void bgrad_kernel(const uint16_t* d, long nvox, int c, float* gb)
 {
-    __shared__ float s[256];
-    const int ch = threadIdx.x % c;
-    const int lanes = 256 / c * c;
-    float acc = 0.f;
-    if (threadIdx.x < lanes)
-        for (long m = (long)blockIdx.x * (256 / c) + threadIdx.x / c; m < nvox; m += (long)gridDim.x * (256 / c))
-            acc += bf2f(d[m * c + ch]);
-    s[threadIdx.x] = threadIdx.x < lanes ? acc : 0.f;
+    __shared__ float s[256 * 8];
+    const int g = c / 8, per = 256 / g;                     // 8-channel groups per voxel, voxels per pass
+    const int cg = threadIdx.x % g, vi = threadIdx.x / g;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (vi < per)
+        for (long m = (long)blockIdx.x * per + vi; m < nvox; m += (long)gridDim.x * per) {
+            const uint4 u = *(const uint4*)(d + m * c + cg * 8);
+            const uint16_t* e = (const uint16_t*)&u;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[j] += bf2f(e[j]);
+        }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[threadIdx.x * 8 + j] = acc[j];
     __syncthreads();
-    if (threadIdx.x < c) {
+    for (int ch = threadIdx.x; ch < c; ch += 256) {
         float t = 0.f;
-        for (int q = threadIdx.x; q < lanes; q += c) t += s[q];
-        atomicAdd(&gb[threadIdx.x], t);
+        for (int v = 0; v < per; ++v) t += s[(v * g + ch / 8) * 8 + ch % 8];
+        atomicAdd(&gb[ch], t);
     }
 }
 
@@ -757,13 +872,21 @@ hipError_t wgrad_launch(tam_dn* n, const uint16_t* x, const uint16_t* dy, int lv
 {
     WgradArgs a{x, dy, gw, n->D[lvl], n->H[lvl], n->W[lvl], cin, cout, 0, 0};
     a.blocks = n->nv[lvl] / 32;
-    // about 96 workgroups per tap (27 taps: ~2600 in all)
-    long per = (a.blocks + 95) / 96;
+    // rows of whole 32-voxel segments: one workgroup per (dz, dy) covers the three dx taps, ~288 per (dz, dy);
+    // otherwise one per tap, ~96 per tap (~2600 workgroups in all either way)
+    // (the 64 x 64 layer keeps one workgroup per tap: three taps' accumulators leave it one wave per SIMD)
+    const bool rows = n->W[lvl] % 32 == 0 && (cin / 16) * (cout / 16) <= 12;
+    const long wgs = rows ? 288 : 96;
+    long per = (a.blocks + wgs - 1) / wgs;
     per = (per + 3) / 4 * 4;
     if (per < 4) per = 4;
     a.per_wg = per;
-    const dim3 grid((unsigned)((a.blocks + per - 1) / per), 27);
-#define WG(CI, CO) hipLaunchKernelGGL((wgrad_kernel<CI, CO>), grid, dim3(256), 0, n->stream, a)
+    const dim3 grid((unsigned)((a.blocks + per - 1) / per), rows ? 9 : 27);
+#define WG(CI, CO)                                                                                     \
+    do {                                                                                               \
+        if (rows) hipLaunchKernelGGL((wgrad3_kernel<CI, CO>), grid, dim3(256), 0, n->stream, a);       \
+        else hipLaunchKernelGGL((wgrad_kernel<CI, CO>), grid, dim3(256), 0, n->stream, a);             \
+    } while (0)
     const int key = (cin / 16) * 16 + cout / 16;
     switch (key) {
     case 1 * 16 + 1: WG(1, 1); break;
@@ -835,7 +958,10 @@ hipError_t backward(tam_dn* n, const float* tgt)
         const Layer& L = n->L[l];
         hipError_t r = wgrad_launch(n, in, dl, L.level, L.cin, L.cout, n->g + L.w_off);
         if (r != hipSuccess) return r;
-        hipLaunchKernelGGL(bgrad_kernel, dim3(grid_of(n->nv[L.level], 256 / L.cout)), dim3(256), 0, n->stream, dl,
+        // at most 1024 blocks: every block ends in `cout` atomics onto the same few addresses, which serialise
+        if (L.cout % 8 != 0 || L.cout > 2048) return hipErrorInvalidValue;
+        unsigned nb = grid_of(n->nv[L.level], 256 / (L.cout / 8));
+        hipLaunchKernelGGL(bgrad_kernel, dim3(nb < 1024u ? nb : 1024u), dim3(256), 0, n->stream, dl,
                            n->nv[L.level], L.cout, n->g + L.b_off);
         return hipGetLastError();
     };
