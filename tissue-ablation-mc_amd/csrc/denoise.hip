@@ -655,7 +655,13 @@ __global__ __launch_bounds__(256) void head_kernel(const uint16_t* x0, const uin
     double l = 0.0, lb = 0.0;
     for (long m = (long)blockIdx.x * 256 + threadIdx.x; m < nvox; m += (long)gridDim.x * 256) {
         float v = bf2f(x0[m * 16]) + hw[16];
-        for (int c = 0; c < 16; ++c) v += hw[c] * bf2f(a[m * 16 + c]);
+        const uint4 u0 = *(const uint4*)(a + m * 16), u1 = *(const uint4*)(a + m * 16 + 8);
+        const uint16_t* e0 = (const uint16_t*)&u0;
+        const uint16_t* e1 = (const uint16_t*)&u1;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v += hw[c] * bf2f(e0[c]);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v += hw[8 + c] * bf2f(e1[c]);
         out[m] = v;
         if (tgt) {
             const float t = tgt[m];
@@ -692,10 +698,18 @@ __global__ __launch_bounds__(256) void head_bwd_kernel(const float* out, const f
     for (long m = (long)blockIdx.x * 256 + threadIdx.x; m < nvox; m += (long)gridDim.x * 256) {
         const float diff = out[m] - tgt[m];
         const float g = (diff > 0.f ? inv : (diff < 0.f ? -inv : 0.f));
-        for (int c = 0; c < 16; ++c) {
-            const float av = bf2f(a[m * 16 + c]);
-            acc[c] += g * av;
-            d9[m * 16 + c] = av > 0.f ? f2bf(g * hw[c]) : (uint16_t)0;
+        uint4 u[2] = {*(const uint4*)(a + m * 16), *(const uint4*)(a + m * 16 + 8)};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            uint16_t* e = (uint16_t*)&u[h];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int c = 8 * h + j;
+                const float av = bf2f(e[j]);
+                acc[c] += g * av;
+                e[j] = av > 0.f ? f2bf(g * hw[c]) : (uint16_t)0;
+            }
+            *(uint4*)(d9 + m * 16 + 8 * h) = u[h];
         }
         acc[16] += g;
     }
@@ -942,7 +956,9 @@ hipError_t forward(tam_dn* n, const float* tgt)
     if ((e = conv(8, n->c8, n->a[8])) != hipSuccess) return e;
     if ((e = conv(9, n->a[8], n->a[9])) != hipSuccess) return e;
     if (tgt) hipLaunchKernelGGL(fill_f64k, dim3(1), dim3(64), 0, n->stream, n->red, 2L, 0.0);
-    hipLaunchKernelGGL(head_kernel, dim3(grid_of(n->nv[0])), dim3(256), 0, n->stream, n->x0, n->a[9],
+    // the head kernels end every workgroup in atomics onto a few addresses: at most 1024 workgroups
+    const unsigned hb = grid_of(n->nv[0]) < 1024u ? grid_of(n->nv[0]) : 1024u;
+    hipLaunchKernelGGL(head_kernel, dim3(hb), dim3(256), 0, n->stream, n->x0, n->a[9],
                        n->p + n->head_off, n->nv[0], n->out, tgt, n->red, n->red + 1);
     return hipGetLastError();
 }
@@ -952,7 +968,8 @@ hipError_t backward(tam_dn* n, const float* tgt)
 {
     hipError_t e;
     hipLaunchKernelGGL(fill_f32k, dim3(grid_of(n->nparam)), dim3(256), 0, n->stream, n->g, n->nparam, 0.f);
-    hipLaunchKernelGGL(head_bwd_kernel, dim3(grid_of(n->nv[0])), dim3(256), 0, n->stream, n->out, tgt, n->a[9],
+    const unsigned hb = grid_of(n->nv[0]) < 1024u ? grid_of(n->nv[0]) : 1024u;
+    hipLaunchKernelGGL(head_bwd_kernel, dim3(hb), dim3(256), 0, n->stream, n->out, tgt, n->a[9],
                        n->p + n->head_off, n->nv[0], n->d[9], n->g + n->head_off);
     auto grads = [&](int l, const uint16_t* in, const uint16_t* dl) {
         const Layer& L = n->L[l];
