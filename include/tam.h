@@ -183,7 +183,9 @@ int tam_mc_run(tam_ctx* ctx, int64_t npackets, uint64_t seed, uint32_t stream, i
  * the beam axis in LDS (FP64; the CU's LDS shared by the blocks per CU, at
  * most the device's per-block limit) and adds the box to the tally once at
  * the end, instead of one device atomic per voxel visit (1 = on, the
- * default).  Each transport launch places the cube's z range where the
+ * default).  With the box on, blocks_per_cu 4 and 8 run as 1 and 2 blocks of
+ * 1024 lanes per CU: the same lanes, one box per 1024 (27^3 voxels at 1 per
+ * CU on MI355X, against four 17^3 copies of 256-lane blocks).  Each transport launch places the cube's z range where the
  * tally it is about to clear (the last run's path), over voxels with
  * mu_t > 0, sums highest inside the cube's x-y footprint (ties: the higher
  * range); with no path yet the cube starts at the top face.

@@ -108,13 +108,14 @@ def test_mc_same_stream_through_a_crater():
 
 def test_mc_launch_shape_does_not_change_results():
     """Packets draw from their own streams whichever lane takes them: every
-    (blocks per CU, random-phase batch, LDS tally box on / off) gives the same
+    (blocks per CU, random-phase batch, LDS tally box on / off; with the box on,
+    4 and 8 run as 1 and 2 blocks of 1024 lanes) gives the same
     integer fates and the same tally up to FP64 summation order.  The tuner
     leaves the tally clear."""
     prob = _two_layer_problem(n=32, packets=300_000)
     g = prob.grid()
     res = []
-    for bpc, batch, box in ((1, 1, 1), (2, 64, 1), (8, 32, 0), (5, 13, 1), (2, 32, 0)):
+    for bpc, batch, box in ((1, 1, 1), (2, 64, 1), (8, 32, 0), (5, 13, 1), (2, 32, 0), (4, 32, 1), (8, 16, 1)):
         _print(f"case: launch bpc={bpc} batch={batch} box={box}")
         g.set_mc_launch(bpc, batch)
         g.set_mc_box(box)

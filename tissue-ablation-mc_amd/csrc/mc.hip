@@ -408,8 +408,10 @@ __global__ __launch_bounds__(256) void mc_kernel(McArgs a) { transport<false>(a,
 
 // The same with the block-private LDS tally box: cleared first, added to the
 // device tally (one atomic per touched voxel) once every wave of the block
-// has drained the queue.
-__global__ __launch_bounds__(256) void mc_box_kernel(McArgs a)
+// has drained the queue.  NT = 256, or 1024: one block of 16 waves per CU
+// holds one box of the CU's whole LDS instead of four quarter-size copies.
+template <int NT>
+__global__ __launch_bounds__(NT) void mc_box_kernel(McArgs a)
 {
     extern __shared__ double box[];
     // geometry: chosen by box_window_kernel (inside the grid, at most the
@@ -421,12 +423,12 @@ __global__ __launch_bounds__(256) void mc_box_kernel(McArgs a)
         g.n[q] = a.box_geom ? __builtin_amdgcn_readfirstlane(a.box_geom[3 + q]) : a.box_n[q];
     }
     const int nbox = g.n[0] * g.n[1] * g.n[2];
-    for (int i = threadIdx.x; i < nbox; i += 256) box[i] = 0.0;
+    for (int i = threadIdx.x; i < nbox; i += NT) box[i] = 0.0;
     __syncthreads();
     transport<true>(a, box, g);
     __syncthreads();
     const uint32_t nxy = (uint32_t)a.nx * (uint32_t)a.ny;
-    for (int i = threadIdx.x; i < nbox; i += 256) {
+    for (int i = threadIdx.x; i < nbox; i += NT) {
         const double s = box[i];
         if (s != 0.0) {
             const int bx = i % g.n[0], by = (i / g.n[0]) % g.n[1], bz = i / (g.n[0] * g.n[1]);
@@ -633,7 +635,9 @@ hipError_t mc_prepare(int dev)
     hipError_t e = hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (e != hipSuccess || lds_blk <= 64 * 1024) return e;
     // allow the box kernels the device's whole per-block LDS (outside any graph capture)
-    e = hipFuncSetAttribute((const void*)mc_box_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_blk);
+    e = hipFuncSetAttribute((const void*)mc_box_kernel<256>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_blk);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)mc_box_kernel<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_blk);
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void*)mc_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_blk);
 }
@@ -657,8 +661,12 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
     // packets from the queue until it is empty, so more blocks than packets
     // only idle
     const int bpc = a.blocks_per_cu > 0 ? a.blocks_per_cu : 8;
-    long blocks = (long)ncu * bpc;
-    const long need = (long)((a.npk + 255) / 256);
+    // with the box on, a multiple of 4 runs as bpc / 4 blocks of 1024 lanes:
+    // the same waves per CU, one box per 16 waves
+    const bool wide = a.box_enable && bpc % 4 == 0;
+    const int lanes = wide ? 1024 : 256, bpc_l = wide ? bpc / 4 : bpc;
+    long blocks = (long)ncu * bpc_l;
+    const long need = (long)((a.npk + lanes - 1) / lanes);
     if (need < blocks) blocks = need;
     // LDS tally box: the resident blocks of a CU share its LDS (160 KiB on
     // MI355X, also the most one block may take), 1 KiB per block kept for
@@ -672,7 +680,7 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
     McArgs b = a;
     b.box_n[0] = b.box_n[1] = b.box_n[2] = 0;
     b.box_geom = nullptr;
-    const long vox = std::min((long)lds_blk - 1024L, (long)lds_cu / bpc - 1024L) / 8L;
+    const long vox = std::min((long)lds_blk - 1024L, (long)lds_cu / bpc_l - 1024L) / 8L;
     BoxPlace bp{};
     bp.sd = std::min((int)std::cbrt((double)vox), std::min(a.nx, a.ny));
     bp.d = bp.sd > 0 ? (int)std::min<long>(a.nz, vox / ((long)bp.sd * bp.sd)) : 0;
@@ -695,7 +703,8 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
             const hipError_t ef = launch_fill_f64(a.tally, nvox, 0.0, s);
             if (ef != hipSuccess) return ef;
         }
-        hipLaunchKernelGGL(mc_box_kernel, dim3((unsigned)blocks), dim3(256), lds, s, b);
+        if (wide) hipLaunchKernelGGL(mc_box_kernel<1024>, dim3((unsigned)blocks), dim3(1024), lds, s, b);
+        else hipLaunchKernelGGL(mc_box_kernel<256>, dim3((unsigned)blocks), dim3(256), lds, s, b);
     } else {
         if (a.clear_tally) {
             const hipError_t ef = launch_fill_f64(a.tally, nvox, 0.0, s);
