@@ -670,8 +670,8 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
     if (need < blocks) blocks = need;
     // LDS tally box: the resident blocks of a CU share its LDS (160 KiB on
     // MI355X, also the most one block may take), 1 KiB per block kept for
-    // the kernel's own use: a cube of 27^3 voxels at 1 block per CU, 21^3 at
-    // 2, 18^2 x 20 at 3, 17^3 at 4, on the beam axis.  Its z range is where
+    // the kernel's own use: a cube of 27^3 voxels at 1 block per CU (1 or 4
+    // blocks_per_cu), 21^3 at 2 (2 or 8), 18^2 x 20 at 3, on the beam axis.  Its z range is where
     // the last run's path was largest, or the top face (where most of the
     // path of a surface beam lies) before any run.  On the c1 skin cycles a
     // cube caught more deposits past the first cycle than flatter (d ~ s/2,
