@@ -384,22 +384,32 @@ __global__ __launch_bounds__(256) void wgrad3_kernel(WgradArgs a)
         const long r = m0 / a.W;
         const int sy = (int)(r % a.H) + dy, sz = (int)(r / a.H) + dz;
         if (sy < 0 || sy >= a.H || sz < 0 || sz >= a.D) continue;      // wave-uniform
-        for (int q = lane; q < 32 * CO / 8; q += 64) {
-            const int v = q / (CO / 8), c8 = (q % (CO / 8)) * 8;
-            const uint4 d = *(const uint4*)(a.dy + (m0 + v) * a.cout + c8);
-            const uint16_t* e = (const uint16_t*)&d;
+        // staging by voxel pairs: each lane loads 8 channels of two neighbouring voxels and writes one dword
+        // (both voxels) per channel
+        for (int q = lane; q < 16 * (CO / 8); q += 64) {
+            const int p = q / (CO / 8), c8 = (q % (CO / 8)) * 8;
+            const uint4 d0 = *(const uint4*)(a.dy + (m0 + 2 * p) * a.cout + c8);
+            const uint4 d1 = *(const uint4*)(a.dy + (m0 + 2 * p + 1) * a.cout + c8);
+            const uint16_t* e0 = (const uint16_t*)&d0;
+            const uint16_t* e1 = (const uint16_t*)&d1;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) sD[wv][c8 + j][v] = e[j];
+            for (int j = 0; j < 8; ++j)
+                *(uint32_t*)&sD[wv][c8 + j][2 * p] = (uint32_t)e0[j] | ((uint32_t)e1[j] << 16);
         }
         const long srow = ((long)sz * a.H + sy) * a.W;
-        for (int q = lane; q < 34 * CI / 8; q += 64) {
-            const int v = q / (CI / 8) - 1, c8 = (q % (CI / 8)) * 8;
-            const int sx = x0 + v;
-            uint4 d = make_uint4(0u, 0u, 0u, 0u);
-            if (sx >= 0 && sx < a.W) d = *(const uint4*)(a.x + (srow + sx) * a.cin + c8);
-            const uint16_t* e = (const uint16_t*)&d;
+        // pairs p = 0 .. 17: voxels x0 + 2p - 2 and x0 + 2p - 1 at elements 2p + 6, 2p + 7 (elements 7 .. 40 are
+        // the ones read; 6 and 41 come along)
+        for (int q = lane; q < 18 * (CI / 8); q += 64) {
+            const int p = q / (CI / 8), c8 = (q % (CI / 8)) * 8;
+            const int sx = x0 + 2 * p - 2;
+            uint4 d0 = make_uint4(0u, 0u, 0u, 0u), d1 = make_uint4(0u, 0u, 0u, 0u);
+            if (sx >= 0 && sx < a.W) d0 = *(const uint4*)(a.x + (srow + sx) * a.cin + c8);
+            if (sx + 1 >= 0 && sx + 1 < a.W) d1 = *(const uint4*)(a.x + (srow + sx + 1) * a.cin + c8);
+            const uint16_t* e0 = (const uint16_t*)&d0;
+            const uint16_t* e1 = (const uint16_t*)&d1;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) sX[wv][c8 + j][8 + v] = e[j];
+            for (int j = 0; j < 8; ++j)
+                *(uint32_t*)&sX[wv][c8 + j][2 * p + 6] = (uint32_t)e0[j] | ((uint32_t)e1[j] << 16);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");         // the slice is written before it is read
         __builtin_amdgcn_wave_barrier();
