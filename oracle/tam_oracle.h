@@ -13,7 +13,7 @@
  * (lower bound 0 in the Fortran, src/subs.f90:66,72 and src/3dFD.f90:254-258).
  *
  * Pinning of each function (how a plausible mistake is caught) is listed in
- * tests/test_oracle_*.py; functions without an independent pin say so here:
+ * tests/test_oracle.py and tests/test_oracle_pins.py:
  *   - or_ran2, or_ref_mc (sourcephCO2 + tauint1), or_stokes and the
  *     thermal-property fits are pinned bit-for-bit against the reference
  *     Fortran compiled from its own sources (oracle/Makefile target `ref`).
@@ -21,9 +21,14 @@
  *     rocRAND's Philox4x32-10 (a different implementation).
  *   - or_mc_run (extended physics) is pinned by Beer-Lambert, the HG moment
  *     <cos theta> = g, the beam radial CDFs, A+R+T=1, normal-incidence Fresnel,
- *     and statistical agreement with the reference-formula variant.
+ *     the multiple-reflection closed form of a pure-absorber slab between two
+ *     mismatched faces (the internal Fresnel reflection of the walk), and
+ *     statistical agreement with the reference-formula variant.
  *   - or_heat_steps is pinned by the discrete Fourier-mode decay factor,
- *     a steady linear profile, an energy balance and a hand-computed 3^3 case.
+ *     a steady linear profile, the steady profile of a convective (Robin) top
+ *     face, the series-resistance steady state of random kappa/rho/c, the
+ *     reciprocity of the update with random properties, and the exact heat
+ *     balance of an insulated box with random conductivity.
  *   - or_arrhenius by the constant-temperature closed form; or_rebuild by a
  *     hand-computed crater and by brute force over tiny grids.
  */

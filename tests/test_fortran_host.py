@@ -57,6 +57,32 @@ def test_driver_setup_matches_python_host(tmp_path):
         assert "no HIP device" in text
 
 
+@needs_host
+def test_driver_short_pulse_step_guard(tmp_path):
+    """A pulse shorter than the explicit step: the Fortran host prints the
+    reference's message and runs with delt = pulse/100, with the run length
+    from the original step (3dFD.f90:102-105, mcpolar.f90:133-140)."""
+    res = tmp_path / "res"
+    res.mkdir()
+    lines = open(os.path.join(GOLD, "input.params")).read().splitlines()
+    lines[10] = "0.01         energyPerPixel (mJ)"
+    lines[13] = "tophat    pulsetype"
+    (res / "input.params").write_text("\n".join(lines) + "\n")
+    r = subprocess.run([os.path.join(FDIR, "mcpolar_gpu"), str(res), str(tmp_path), "24", "0"],
+                       capture_output=True, text=True, timeout=120)
+    text = r.stdout + r.stderr
+    print(text)
+    assert "pulselength smaller than timestep, adjusting..." in text
+    delt, total, pulse = (float(x) for x in re.search(r"# delt\s+(\S+) total_time\s+(\S+) pulse\s+(\S+)",
+                                                       text).groups())
+    inp = configs.read_input_params(str(res / "input.params"))
+    prob = configs.reference_default(inp, 24)
+    assert math.isclose(delt, driver.step_guard(prob.pulse_length, prob.params.delt), rel_tol=1e-15)
+    assert math.isclose(delt, prob.pulse_length / 100.0, rel_tol=1e-15)
+    real_pulse, total_py = driver.run_length(inp, prob.pulse_length, prob.params.delt)
+    assert math.isclose(total, total_py, rel_tol=1e-15) and math.isclose(pulse, real_pulse, rel_tol=1e-15)
+
+
 def test_output_tag_follows_the_reference_formatting():
     """str(int, len) zero-pads a short integer and keeps the leading digits of
     a long one; str(real, len) keeps the leading characters of f100.16."""

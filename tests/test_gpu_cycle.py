@@ -27,9 +27,13 @@ def _skin(steps):
 
 # (steps per cycle, stepwise launch shape, graph launch shape, transport
 # blocks per CU); shape 8 runs two steps per launch, 4 one (None: the
-# default shape); 1 block per CU gives the largest LDS tally box (160 KiB)
+# default shape); 1 block per CU gives the largest LDS tally box (160 KiB).
+# On shape 8 a replay makes ceil(steps / 2) ping-pong swaps: 2, 3, 6 and 30
+# steps give an odd count of launches from an even step count or the reverse,
+# so the host's buffer parity after a replay must follow the launches.
 @pytest.mark.parametrize("steps,shape_a,shape_b,bpc", [(60, None, None, None), (61, None, None, None),
-                                                       (61, 4, 8, None), (60, 8, 4, 1)])
+                                                       (61, 4, 8, None), (60, 8, 4, 1), (2, 8, 8, None),
+                                                       (3, 8, 8, None), (6, 4, 8, None), (30, 8, 8, None)])
 def test_graph_cycle_matches_stepwise_fp32(steps, shape_a, shape_b, bpc):
     _print(f"case: 40^3 skin, {steps} steps per cycle, 4 cycles, graph vs stepwise, shapes {shape_a} {shape_b}, "
            f"transport blocks per CU {bpc}")
@@ -55,7 +59,8 @@ def test_graph_cycle_matches_stepwise_fp32(steps, shape_a, shape_b, bpc):
         ma, mb = a.grid.download(lib.FIELD_MAT), b.grid.download(lib.FIELD_MAT)
         assert np.array_equal(ma, mb), cyc
         _print(f"cycle {cyc}: max T {Ta.max():.2f} K, removed {int((ma == len(prob.table)).sum())}")
-    assert (ma == len(prob.table)).any(), "the case should ablate"
+    if steps >= 60:
+        assert (ma == len(prob.table)).any(), "the case should ablate"
     a.close()
     b.close()
 

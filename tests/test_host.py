@@ -75,3 +75,23 @@ def test_batch_manifest_is_the_committed_one():
     assert len(m) == 64 and {x["edge"] for x in m} <= {64, 96, 128, 160}
     assert all(0.1 <= x["mua"] <= 10 and 50 <= x["mus"] <= 300 and 0.7 <= x["g"] <= 0.95 for x in m)
     assert all(1e6 <= x["packets"] <= 1e7 for x in m)
+
+
+def test_short_pulse_step_guard():
+    """heat_sim_3D's guard (3dFD.f90:102-105): a pulse shorter than the
+    explicit step shortens the step to pulse/100; the run length is set
+    before it, from the original step (mcpolar.f90:133-140)."""
+    from tam_amd import driver
+    inp = dict(configs.REFERENCE_INPUT, energyPerPixel=0.01, pulsetype="tophat")
+    prob = configs.reference_default(inp, 24)
+    delt0, pl = prob.params.delt, prob.pulse_length
+    assert math.isclose(pl, 0.01e-3 * 81 / 70.0, rel_tol=1e-15) and pl < delt0
+    assert driver.step_guard(pl, delt0) == pl / 100.0
+    assert driver.step_guard(pl, 0.5 * pl) == 0.5 * pl        # a longer pulse keeps the step
+    assert driver.step_guard(pl, pl) == pl                    # strict comparison
+    # the run length uses the original step: tophat, total_time 2 s is kept
+    real_pulse, total = driver.run_length(inp, pl, delt0)
+    assert real_pulse == pl and total == inp["total_time"]
+    # the reference default (a 0.46 s pulse) is not affected
+    ref = configs.reference_default(n=24)
+    assert driver.step_guard(ref.pulse_length, ref.params.delt) == ref.params.delt

@@ -330,9 +330,11 @@ size_t cycle_bytes(int loops) { return cycle_ta_off(loops) + (size_t)loops * 8; 
 
 // One coupling cycle on c->stream, reading the per-cycle values from the
 // device parameter block.  Swaps the host's ping-pong pointers as the steps
-// are enqueued (the caller restores them after a capture).
-int enqueue_cycle(tam_ctx* c)
+// are enqueued (the caller restores them after a capture) and counts the
+// swaps in *swaps: one per launch, so a two-step launch swaps once.
+int enqueue_cycle(tam_ctx* c, int* swaps)
 {
+    *swaps = 0;
     const CycleHdr* hdr = (const CycleHdr*)c->cyc_dev;
     const float* lon = (const float*)(c->cyc_dev + cycle_lon_off());
     const double* ta = (const double*)(c->cyc_dev + cycle_ta_off(c->cyc_loops));
@@ -358,6 +360,7 @@ int enqueue_cycle(tam_ctx* c)
             a64.dyn_tafter = ta + k;
             HIPCHK(c, launch_heat64_step(a64, c->cur64, c->nxt64, 0.0, 0.0, boil, robin, dmg, 1, s));
             std::swap(c->cur64, c->nxt64);
+            ++*swaps;
         } else {
             a.dyn_lon = lon + k;
             a.dyn_tafter = ta + k;
@@ -370,6 +373,7 @@ int enqueue_cycle(tam_ctx* c)
                                            c->tile, c->kz, s));
             }
             std::swap(c->cur, c->nxt);
+            ++*swaps;
         }
     }
     // Arrhenius at the reference cadence (mcpolar.f90:181)
@@ -393,7 +397,8 @@ int capture_cycle(tam_ctx* c, int par)
     float* cur = c->cur; float* nxt = c->nxt;
     double* cur64 = c->cur64; double* nxt64 = c->nxt64;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-    const int rc = enqueue_cycle(c);
+    int swaps = 0;
+    const int rc = enqueue_cycle(c, &swaps);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &g);
     c->cur = cur; c->nxt = nxt; c->cur64 = cur64; c->nxt64 = nxt64;
@@ -408,6 +413,7 @@ int capture_cycle(tam_ctx* c, int par)
         c->cyc_exec[par] = nullptr;
         return hip_fail(c, "hipGraphInstantiate", ei);
     }
+    c->cyc_swap[par] = swaps & 1;
     return TAM_OK;
 }
 
@@ -934,7 +940,7 @@ int tam_cycle(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream, doub
     c->cyc_slot = (slot + 1) % 4;
     HIPCHK(c, hipGraphLaunch(c->cyc_exec[par], c->stream));
     if (timed) HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    if (loops & 1) {
+    if (c->cyc_swap[par]) {                        // the replay's net ping-pong swap
         std::swap(c->cur, c->nxt);
         std::swap(c->cur64, c->nxt64);
     }

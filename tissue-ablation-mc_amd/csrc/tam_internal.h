@@ -255,6 +255,7 @@ struct tam_ctx {
     // captured coupling cycle (tam_cycle): one executable graph per starting
     // buffer of the temperature ping-pong (an odd step count swaps them)
     hipGraphExec_t cyc_exec[2];
+    int cyc_swap[2];            // ping-pong swaps (mod 2) one replay of cyc_exec[p] makes
     long cyc_npk;               // capture key: packet capacity, steps, launch shapes, flags
     int cyc_loops, cyc_arr;
     uint64_t cyc_seed;

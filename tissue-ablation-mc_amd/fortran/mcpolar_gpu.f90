@@ -32,12 +32,14 @@ program mcpolar_gpu
     type(c_ptr)        :: ctx
     double precision, allocatable :: temp(:,:,:), kappa(:,:,:), density(:,:,:), heatcap(:,:,:)
     double precision, allocatable, target :: rhokap(:,:,:)
+    double precision, allocatable :: coeff(:,:,:)
     integer(c_int8_t), allocatable :: mat(:,:,:)
     character(len=512) :: arg, resdir, outdir
     integer            :: n, max_cycles, cycles, nargs, inert
     integer(c_int)     :: rc
     integer(c_int64_t) :: npk, seed
-    double precision   :: pulse_length, real_pulse, total_time, pw, scale
+    double precision   :: pulse_length, real_pulse, total_time, pw, scale, delt0
+    logical            :: short_pulse
     logical            :: ramp_down
     integer(c_int64_t) :: t0, t1, rate
 
@@ -69,6 +71,8 @@ program mcpolar_gpu
              heatcap(0:n+1, 0:n+1, 0:n+1), rhokap(0:n+1, 0:n+1, 0:n+1), mat(n, n, n))
     call reference_setup(inp, n, p, temp, kappa, density, heatcap, rhokap, mat, table, pulse_length)
     call run_length(inp, pulse_length, p%delt, real_pulse, total_time)
+    delt0 = p%delt
+    short_pulse = step_guard(pulse_length, p%delt)
     p%removed_inert = int(inert, c_int32_t)
     write(*, '(a, i0, a, i0, a)') "# photons per cycle ", inp%nphotons, ", grid ", n, "^3"
     write(*, '(a, es23.16, a, es23.16, a, es23.16)') "# delt ", p%delt, " total_time ", total_time, &
@@ -79,6 +83,14 @@ program mcpolar_gpu
     call tam_check(tam_set_optics(ctx, mat, table, 1_c_int32_t, c_loc(rhokap)), ctx, "tam_set_optics")
     call tam_check(tam_set_thermal(ctx, temp, kappa, density, heatcap, c_null_ptr, c_null_ptr), ctx, &
                    "tam_set_thermal")
+    if (short_pulse) then
+        ! the first cycle's source coefficient keeps the original step
+        allocate(coeff(0:n+1, 0:n+1, 0:n+1))
+        coeff = 0.d0
+        coeff(1:n, 1:n, 1:n) = delt0 / (density(1:n, 1:n, 1:n) * heatcap(1:n, 1:n, 1:n))
+        call tam_check(tam_upload(ctx, TAM_FIELD_COEFF, coeff), ctx, "tam_upload")
+        deallocate(coeff)
+    end if
 
     pulse = tam_pulse(time=0.d0, pulseCount=0.d0, repetitionCount=0.d0, repetitionRate_1=inp%repetitionRate_1, &
                       realPulseLength=real_pulse, laserOn=1.d0, laser_flag=1, pulsesDone=0, &

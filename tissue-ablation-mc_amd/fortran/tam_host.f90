@@ -192,6 +192,24 @@ contains
         end if
     end subroutine run_length
 
+    ! heat_sim_3D's guard (3dFD.f90:102-105): a pulse shorter than one
+    ! explicit step shortens the step to a hundredth of the pulse.  The
+    ! reference applies it on the first heat call, after mcpolar has set the
+    ! run length from the original step (mcpolar.f90:133-140), and
+    ! initThermalCoeff has already set coeff = delt/(rho c) from the original
+    ! step; setupThermalCoeff rebuilds coeff from the new one after the first
+    ! cycle (3dFD.f90:339-345).  Returns .true. when the step changed.
+    logical function step_guard(pulse_length, delt)
+        double precision, intent(in)    :: pulse_length
+        double precision, intent(inout) :: delt
+
+        step_guard = pulse_length < delt
+        if (step_guard) then
+            print *, "pulselength smaller than timestep, adjusting..."
+            delt = pulse_length / 100.d0
+        end if
+    end function step_guard
+
     ! ---- file names: the reference formats numbers through f100.16 / I100.1
     ! records, left-adjusts them and keeps the leading `w` characters (with
     ! zero padding on the left for a short integer)

@@ -31,6 +31,17 @@ def run_length(inp, pulse_length, delt):
     return real_pulse, total
 
 
+def step_guard(pulse_length, delt):
+    """heat_sim_3D's guard (3dFD.f90:102-105): a pulse shorter than one
+    explicit step shortens the step to a hundredth of the pulse.  The
+    reference applies it on the first heat call, after the run length was
+    set from the original step (mcpolar.f90:133-140); the coefficient
+    delt/(rho c) of the first cycle's source still holds the original step
+    (initThermalCoeff, 3dFD.f90:282-283) until setupThermalCoeff rebuilds it
+    (3dFD.f90:339-345).  Returns the step the heat loop uses."""
+    return pulse_length / 100.0 if pulse_length < delt else delt
+
+
 class PowerFunction:
     """getPwrGaussian / TopHat / Triangular (3dFD.f90:365-421)."""
 
@@ -77,7 +88,14 @@ def reference_run(inp=None, n=80, max_cycles=None, seed=SEED):
     prob = configs.reference_default(inp, n)
     p = prob.params
     real_pulse, total = run_length(inp, prob.pulse_length, p.delt)
+    delt0 = p.delt
+    p.delt = step_guard(prob.pulse_length, delt0)
     g = prob.grid()
+    if p.delt != delt0:
+        # the first cycle's source coefficient keeps the original step
+        coeff = np.zeros_like(prob.density)
+        coeff[1:-1, 1:-1, 1:-1] = delt0 / (prob.density * prob.heatcap)[1:-1, 1:-1, 1:-1]
+        g.upload(lib.FIELD_COEFF, coeff)
     pulse = lib.Pulse()
     pulse.repetitionRate_1 = inp["repetitionRate_1"]
     pulse.realPulseLength = real_pulse
