@@ -1,0 +1,184 @@
+"""The batched and the coupled-denoiser configurations against the oracle and
+numpy, through the C-ABI:
+
+  * batched instances (the 64-tissue manifest): one instance of each grid edge
+    {64, 96, 128, 160} with its manifest optics, run in one batched transport
+    launch and batched stencil steps, each instance against the oracle on its
+    own Philox stream (fate fractions 1e-3 absolute, tally L1 1e-2) and the
+    oracle's FP64 stencil fed the GPU's jmean (1e-5 relative);
+  * the chosen form of the coupled run: tam_tally_blocks against numpy block
+    sums of the downloaded tally (1e-12 relative); tam_dn_apply against
+    tam_dn_forward_host on the same features (the prediction, back as path
+    length); and the residual fallback: with tolerance 0 every cycle takes the
+    high-count branch, so the run equals the plain form exactly.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from tam_amd import configs, coupled, lib
+from tam_amd.denoise import Denoiser
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tissue-ablation-mc_amd",
+                       "weights", "denoiser_c3.f32")
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def _manifest_pick():
+    man = json.load(open(os.path.join(os.path.dirname(configs.__file__), "batch_manifest.json")))
+    pick = []
+    for edge in (64, 96, 128, 160):
+        i = next(i for i, m in enumerate(man) if m["edge"] == edge)
+        pick.append((i, dict(man[i], packets=20_000)))
+    return pick
+
+
+def _oracle_heat(prob, g, jm, steps):
+    p = prob.params
+    T = np.ascontiguousarray(g.download(lib.FIELD_TEMP))
+    arrs = [np.ascontiguousarray(g.download(f)) for f in
+            (lib.FIELD_KAPPA, lib.FIELD_DENSITY, lib.FIELD_HEATCAP, lib.FIELD_COEFF, lib.FIELD_RHOKAP)]
+    jm = np.ascontiguousarray(jm)
+    Q = np.zeros(prob.mat.shape)
+    h = O.Heat()
+    h.nx, h.ny, h.nz = p.nx, p.ny, p.nz
+    h.dx, h.dy, h.dz = p.dx_m, p.dy_m, p.dz_m
+    h.delt, h.dscale = p.delt, p.dscale
+    h.temp = T.ctypes.data
+    h.kappa, h.density, h.heatcap, h.coeff, h.rhokap = [a.ctypes.data for a in arrs]
+    h.jmean, h.Q = jm.ctypes.data, Q.ctypes.data
+    h.volumeVoxel, h.massVoxel, h.QVapor = p.volume_voxel, p.mass_voxel, p.qvapor
+    h.boil, h.robin_top, h.h_top, h.T_amb = p.boil, p.robin_top, p.h_top, p.t_amb
+    h.removed_inert = p.removed_inert
+    assert O.lib().or_heat_steps(C.byref(h), steps, None) == 0
+    return T
+
+
+def test_batched_manifest_instances_follow_the_oracle():
+    pick = _manifest_pick()
+    man = [m for _, m in pick]
+    probs = configs.batch_problems(man, steps=20)
+    gs = [p.grid() for p in probs]
+    b = lib.Batch(gs)
+    b.set_launch(7, 8, 4, 32)
+    _print("case: batched instances", [(i, m["edge"]) for i, m in pick])
+    npk = [m["packets"] for m in man]
+    st = b.mc_run(npk, seed=7, stream0=100)
+    jms = []
+    for k, (prob, g) in enumerate(zip(probs, gs)):
+        p = prob.params
+        tally = g.download(lib.FIELD_TALLY)
+        orc = O.MCProblem(p.nx, p.ny, p.nz, p.xmax, p.ymax, p.zmax, prob.mat, prob.table,
+                          n_ext=(0, 0, 0, 0, p.n_ext_zlo, p.n_ext_zhi), beam=p.beam, radius=p.beam_radius,
+                          rng=0, seed=7, stream=100 + k)
+        t_or, ost = orc.run(npk[k])
+        fg = st[k].fractions()
+        fo = {"absorbed": ost.absorbed / npk[k], "specular": ost.specular / npk[k],
+              "diffuse_reflected": ost.escaped[5] / npk[k], "transmitted": ost.escaped[4] / npk[k],
+              "lateral": sum(ost.escaped[:4]) / npk[k], "missed": ost.missed / npk[k]}
+        rel = np.abs(tally - t_or).sum() / t_or.sum()
+        _print(f"instance edge {p.nx}: absorbed gpu {fg['absorbed']:.4f} oracle {fo['absorbed']:.4f}, "
+               f"tally L1 {rel:.2e}")
+        for key in fg:
+            assert abs(fg[key] - fo[key]) <= 1e-3, (p.nx, key)
+        assert rel < 1e-2
+        g.normalise(prob.scale)
+        jms.append(g.download(lib.FIELD_JMEAN))
+    T0 = [_oracle_heat(prob, g, jm, 20) for prob, g, jm in zip(probs, gs, jms)]
+    b.heat_run(20)
+    b.close()
+    for prob, g, To in zip(probs, gs, T0):
+        Tg = g.download(lib.FIELD_TEMP)
+        inner = (slice(1, -1),) * 3
+        relT = (np.abs(Tg - To)[inner] / To[inner]).max()
+        _print(f"instance edge {prob.params.nx}: heat max rel {relT:.2e}, max rise {(Tg - To.min()).max():.3g}")
+        assert relT <= 1e-5
+        assert Tg.max() > configs.T_BODY + 1e-4
+        g.close()
+
+
+# ------------------------------------------------------------ coupled c4 --
+
+def _skin64(packets):
+    return configs.config4_tissue(n=(64, 64, 64), ext=(0.5, 0.5, 0.25), packets=packets, steps=40, cycles=2,
+                                  power=1.0)
+
+
+def test_tally_blocks_are_numpy_block_sums():
+    prob = _skin64(200_000)
+    g = prob.grid()
+    g.mc_run(prob.packets, seed=3)
+    t = g.download(lib.FIELD_TALLY)
+    for k in (16, 8, 4):
+        B = g.tally_blocks(k)
+        ref = t.reshape(64 // k, k, 64 // k, k, 64 // k, k).sum(axis=(1, 3, 5))
+        np.testing.assert_allclose(B.reshape(ref.shape), ref, rtol=1e-12, atol=0)
+    with pytest.raises(lib.TamError):
+        g.tally_blocks(5)                                     # must divide every edge
+    g.close()
+
+
+def test_dn_apply_matches_forward_on_host_features():
+    """tam_dn_apply builds the features on the device from the tally and the
+    optical maps, runs the network and writes the prediction back as path
+    length: the same as the host forward on features built in numpy (FP32
+    arithmetic as on the device; BF16 inputs), to 1e-3 relative in fluence."""
+    prob = _skin64(100_000)
+    p = prob.params
+    g = prob.grid()
+    g.mc_run(prob.packets, seed=4)
+    tally = g.download(lib.FIELD_TALLY)
+    dn = Denoiser((64, 64, 64), capacity=0)
+    dn.set_params(np.fromfile(WEIGHTS, dtype=np.float32))
+    vcm = p.volume_voxel * 1e6
+    tab = np.array(prob.table, dtype=np.float64)
+    mat = prob.mat
+    mut = (tab[mat, 0] + tab[mat, 1]).astype(np.float32)
+    alb = (tab[:, 1] / (tab[:, 0] + tab[:, 1])).astype(np.float32)[mat]
+    f = np.stack([np.log1p(tally * (1.0 / (prob.packets * vcm))).astype(np.float32),
+                  np.log10(np.maximum(mut * (np.float32(1) - alb), np.float32(1e-3))),
+                  np.log10(np.maximum(mut * alb, np.float32(1e-2))) - np.float32(2),
+                  np.float32(5) * (tab[mat, 2].astype(np.float32) - np.float32(0.8))]).astype(np.float32)
+    pred = dn.forward(f)
+    expect = np.maximum(np.expm1(pred.astype(np.float64)), 0.0) * (prob.packets * vcm)
+    dn.apply(g, prob.packets)
+    got = g.download(lib.FIELD_TALLY)
+    dn.close()
+    g.close()
+    sel = expect > 1e-2 * expect.max()
+    rel = np.abs(got[sel] - expect[sel]) / expect[sel]
+    _print("apply vs host forward: median rel", np.median(rel), "max rel", rel.max(), "voxels", sel.sum())
+    assert np.median(rel) < 1e-5 and np.percentile(rel, 99.9) < 1e-3
+    # the denoised field is not the noisy input
+    assert np.median(np.abs(got[sel] - tally[sel]) / tally[sel].clip(1e-30)) > 1e-3
+
+
+def test_residual_fallback_takes_the_high_count_branch():
+    """Tolerance 0: every cycle's residual exceeds it, so each cycle runs the
+    high-count transport on the same stream as the plain form; the final
+    temperature field equals the plain run's bit for bit (same kernels, same
+    launch shapes), and every cycle is counted as a fallback."""
+    prob = _skin64(100_000)
+    dn = Denoiser((64, 64, 64), capacity=0)
+    dn.set_params(np.fromfile(WEIGHTS, dtype=np.float32))
+    kw = dict(cycles=2, steps=40, hi=100_000, lo=10_000, tune=False)
+    plain = coupled.run_form("plain", prob, **kw)
+    forced = coupled.run_form("chosen", prob, dn=dn, tol=0.0, **kw)
+    loose = coupled.run_form("chosen", prob, dn=dn, tol=1e9, **kw)
+    dn.close()
+    assert forced["fallbacks"] == 2 and all(c.get("fallback") for c in forced["cycles"])
+    assert [c["npk"] for c in forced["cycles"]] == [100_000, 100_000]
+    assert np.array_equal(forced["T"], plain["T"])
+    assert loose["fallbacks"] == 0 and [c["npk"] for c in loose["cycles"]] == [10_000, 10_000]
+    rep = coupled.compare(prob, plain, loose)
+    _print("loose vs plain", {k: v for k, v in rep.items() if not isinstance(v, list)})
+    assert rep["peak_T_error"] >= 0.0 and np.isfinite(rep["p99_abs_T_diff_tissue"])

@@ -84,6 +84,50 @@ __device__ __forceinline__ TileId xcd_tile()
     return tl;
 }
 
+// One step's damage at interior voxel v, dT = tn after the step
+// (3dFD.f90:450-461): Omega grows inside the Arrhenius window and the
+// threshold chain runs there; outside it the chain runs where a threshold is
+// still due from the previous step (chk: the plane has such a voxel).  Omega
+// and the mask come from the prefetch when pf.  Returns true when a
+// threshold is still due after this step.
+__device__ __forceinline__ bool damage_one(const HeatArgs& a, long v, long nv, float tn, double t_after, bool chk,
+                                           bool pf, double om_pf, unsigned tm_pf)
+{
+    if (tn >= a.dT_lo_test && tn < a.dT_hi_test) {
+        const double T = a.t_ref + (double)tn;
+        if (T >= a.arr_Tlo && T < a.arr_Thi) {
+            // rate A exp(-dE/RT) = exp(ln A - dE/RT): exponent formed in
+            // FP64 (both terms ~230), exponentiated in FP32 (|arg| < ~40,
+            // <= 2e-6 relative), accumulated in FP64
+            const float arg = (float)(a.lnA - a.arr_dE / (a.arr_R * T));
+            const double om = (pf ? om_pf : a.omega[v]) + a.delt * (double)expf(arg);
+            a.omega[v] = om;
+            // the threshold chain can only fire once Omega reaches the lowest
+            // threshold (and nothing is due below it)
+            if (om < a.thr_min) return false;
+            const unsigned m = thres_chain(a.thres, a.tmask, v, nv, om, a.thr0, a.thr1, a.thr2, t_after,
+                                           pf ? tm_pf : a.tmask[v]);
+            return (m & TAM_TMASK_DUE) != 0u;
+        }
+    }
+    if (chk) {
+        const unsigned m0 = pf ? tm_pf : a.tmask[v];
+        if (m0 & TAM_TMASK_DUE) {
+            const unsigned m = thres_chain(a.thres, a.tmask, v, nv, pf ? om_pf : a.omega[v], a.thr0, a.thr1, a.thr2,
+                                           t_after, m0);
+            return (m & TAM_TMASK_DUE) != 0u;
+        }
+    }
+    return false;
+}
+
+// Step id of a launch: by value, or the graph replay's first step id plus
+// the launch's offset in the cycle.
+__device__ __forceinline__ uint32_t step_id(const HeatArgs& a)
+{
+    return a.dyn_sid ? *a.dyn_sid + a.sid : a.sid;
+}
+
 template <int TX, int TY, bool LUT>
 __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const float* __restrict__ Tin,
                                                          float* __restrict__ Tout, float lon, double t_after,
@@ -166,6 +210,7 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
     const long nxy = (long)nx * ny;
     long v = (long)(ic - 1) + (long)nx * (jc - 1) + nxy * (k0 - 1);
     const float wlo = a.dT_lo_test - a.dT_win_pad, whi = a.dT_hi_test + a.dT_win_pad;
+    const uint32_t sid = step_id(a);
     bool pom = damage && in && tc >= wlo && tc < whi;
     double om_pf = 0.0;
     unsigned tm_pf = 0;
@@ -228,6 +273,7 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
             }
         }
         __syncthreads();
+        const bool chk = damage && plane_due(a.pflag, nz, sid, k);
         if (!in) continue;
         // ---- update of voxel (i, j, k)
         const float txm = sT[ty + 1][tx], txp = sT[ty + 1][tx + 2];
@@ -271,21 +317,9 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
             }
         }
         Tout[c] = tn;
-        if (damage && tn >= a.dT_lo_test && tn < a.dT_hi_test) {   // 3dFD.f90:450-461, every step
-            const double T = a.t_ref + (double)tn;
-            if (T >= a.arr_Tlo && T < a.arr_Thi) {
-                const long nv = nxy * nz;
-                // rate A exp(-dE/RT) = exp(ln A - dE/RT): exponent formed in
-                // FP64 (both terms ~230), exponentiated in FP32 (|arg| < ~40,
-                // <= 2e-6 relative), accumulated in FP64
-                const float arg = (float)(a.lnA - a.arr_dE / (a.arr_R * T));
-                const double om = (pom_k ? om_k : a.omega[v]) + a.delt * (double)expf(arg);
-                a.omega[v] = om;
-                // the threshold chain can only fire once Omega reaches the lowest threshold
-                if (om >= a.thr_min)
-                    thres_chain(a.thres, a.tmask, v, nv, om, a.thr0, a.thr1, a.thr2, t_after,
-                                pom_k ? tmask_k : a.tmask[v]);
-            }
+        if (damage) {                                                // 3dFD.f90:450-461, every step
+            const bool due = damage_one(a, v, nxy * nz, tn, t_after, chk, pom_k, om_k, tmask_k);
+            mark_plane_due(a.pflag, nz, sid + 1u, k, due);
         }
         if (energy && (bnd_ij || k == 1 || k == nz)) {
             float e = 0.f;
@@ -362,6 +396,7 @@ __device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __re
         gzm = gza[c - pxy];
     }
     float e_acc = 0.f;
+    const uint32_t sid = step_id(a);
     for (int k = k0; k < k1; ++k, c += pxy, v += nxy) {
         const float tp0 = Tin[c + pxy];
         const float txm = Tin[c - 1], txp = Tin[c + 1], tym = Tin[c - px], typ = Tin[c + px];
@@ -408,6 +443,7 @@ __device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __re
         }
         const float inc = face_sum(tc, gxm, txm, gxp, txp, gym, tym, gyp, typ, gzm, tm, gzp, tp);
         float tn = step_update(tc, inc, lon, s_k);
+        bool due = false;
         if (in) {
             if (tc < a.dT_neg) a.flags[0] = 1;                   // 3dFD.f90:179-182
             if (boil && tc >= a.dT_boil_test) {                   // 3dFD.f90:167-172
@@ -423,18 +459,8 @@ __device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __re
                 }
             }
             Tout[c] = tn;
-            if (damage && tn >= a.dT_lo_test && tn < a.dT_hi_test) {  // 3dFD.f90:450-461
-                const double T = a.t_ref + (double)tn;
-                if (T >= a.arr_Tlo && T < a.arr_Thi) {
-                    const long nv = nxy * nz;
-                    const float arg = (float)(a.lnA - a.arr_dE / (a.arr_R * T));
-                    const double om = (pom ? om_pf : a.omega[v]) + a.delt * (double)expf(arg);
-                    a.omega[v] = om;
-                    if (om >= a.thr_min)
-                        thres_chain(a.thres, a.tmask, v, nv, om, a.thr0, a.thr1, a.thr2, t_after,
-                                    pom ? tm_pf : a.tmask[v]);
-                }
-            }
+            if (damage)                                                  // 3dFD.f90:450-461
+                due = damage_one(a, v, nxy * nz, tn, t_after, plane_due(a.pflag, nz, sid, k), pom, om_pf, tm_pf);
             if (energy && (bnd_ij || k == 1 || k == nz)) {
                 float e = 0.f;
                 if (i == 1) e += gxm * (txm - tc);
@@ -446,6 +472,7 @@ __device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __re
                 e_acc -= (LUT ? sLut[768 + cc] : (float)(a.heatcap[c] * a.mass_voxel)) * e;
             }
         }
+        if (damage) mark_plane_due(a.pflag, nz, sid + 1u, k, due);
         tm = tc;
         tc = tp0;
         if (LUT) {
@@ -484,11 +511,13 @@ __global__ __launch_bounds__(TX* TY) void stencil_direct_kernel(const HeatArgs a
 // One step's damage at interior voxel v, dT = tn after the step
 // (3dFD.f90:450-461, the same arithmetic as the one-step kernels).  Omega and
 // the threshold mask are loaded on first use (from the prefetch when pom)
-// and kept in registers across the two steps of a launch; `have` says that
-// Omega changed and must be stored.
-__device__ __forceinline__ void damage_step(const HeatArgs& a, long v, long nv, float tn, double t_after, bool pom,
-                                            double om_pf, unsigned tm_pf, double& om, bool& have, unsigned& m,
-                                            bool& have_m)
+// and kept in registers across the two steps of a launch; `dirty` says that
+// Omega changed and must be stored.  Outside the window the chain runs where
+// a threshold is due: from the previous launch when chk (the plane has such a
+// voxel), or from the launch's first step (the mask is then in registers).
+__device__ __forceinline__ void damage_step(const HeatArgs& a, long v, long nv, float tn, double t_after, bool chk,
+                                            bool pom, double om_pf, unsigned tm_pf, double& om, bool& have,
+                                            bool& dirty, unsigned& m, bool& have_m)
 {
     if (tn >= a.dT_lo_test && tn < a.dT_hi_test) {
         const double T = a.t_ref + (double)tn;
@@ -499,6 +528,7 @@ __device__ __forceinline__ void damage_step(const HeatArgs& a, long v, long nv, 
                 have = true;
             }
             om = om + a.delt * (double)expf(arg);
+            dirty = true;
             if (om >= a.thr_min) {
                 if (!have_m) {
                     m = pom ? tm_pf : a.tmask[v];
@@ -506,6 +536,20 @@ __device__ __forceinline__ void damage_step(const HeatArgs& a, long v, long nv, 
                 }
                 m = thres_chain(a.thres, a.tmask, v, nv, om, a.thr0, a.thr1, a.thr2, t_after, m);
             }
+            return;
+        }
+    }
+    if (chk || have_m) {
+        if (!have_m) {
+            m = pom ? tm_pf : a.tmask[v];
+            have_m = true;
+        }
+        if (m & TAM_TMASK_DUE) {
+            if (!have) {
+                om = pom ? om_pf : a.omega[v];
+                have = true;
+            }
+            m = thres_chain(a.thres, a.tmask, v, nv, om, a.thr0, a.thr1, a.thr2, t_after, m);
         }
     }
 }
@@ -562,6 +606,7 @@ __device__ __forceinline__ void t2_body(const HeatArgs& a, const float* __restri
     __syncthreads();
 
     const long nxy = (long)nx * ny, nv = nxy * nz;
+    const uint32_t sid = step_id(a);                             // id of the launch's first step
     const long vcol = (long)(i - 1) + (long)nx * (j - 1);        // interior index of the column (own lanes)
     const float wlo = a.dT_lo_test - a.dT_win_pad, whi = a.dT_hi_test + a.dT_win_pad;
     // first step at plane p: dT0 at p-1, p (z0m, z0c), classes at p-1, p
@@ -654,6 +699,8 @@ __device__ __forceinline__ void t2_body(const HeatArgs& a, const float* __restri
             }
         }
         // ---- second step at plane k = p-1 (dT1 of plane k is in sT1[k & 1])
+        const bool chk = step2 && damage && plane_due(a.pflag, nz, sid, k);
+        bool due2 = false;
         if (step2 && own) {
             const float xm = sT1[k & 1][ty][tx - 1], xp = sT1[k & 1][ty][tx + 1];
             const float ym = sT1[k & 1][ty - 1][tx], yp = sT1[k & 1][ty + 1][tx];
@@ -669,10 +716,11 @@ __device__ __forceinline__ void t2_body(const HeatArgs& a, const float* __restri
             if (damage) {                                       // step 1, then step 2
                 double om = 0.0;
                 unsigned m = 0;
-                bool have = false, have_m = false;
-                damage_step(a, v, nv, u1c, ta0, pom, om_pf, tm_pf, om, have, m, have_m);
-                damage_step(a, v, nv, tn, ta1, pom, om_pf, tm_pf, om, have, m, have_m);
-                if (have) a.omega[v] = om;
+                bool have = false, dirty = false, have_m = false;
+                damage_step(a, v, nv, u1c, ta0, chk, pom, om_pf, tm_pf, om, have, dirty, m, have_m);
+                damage_step(a, v, nv, tn, ta1, false, pom, om_pf, tm_pf, om, have, dirty, m, have_m);
+                if (dirty) a.omega[v] = om;
+                due2 = have_m && (m & TAM_TMASK_DUE);
             }
             if (energy && (bnd_ij || k == 1 || k == nz)) {
                 float e = 0.f;
@@ -685,6 +733,7 @@ __device__ __forceinline__ void t2_body(const HeatArgs& a, const float* __restri
                 e_acc -= sLut[768 + Ck] * e;
             }
         }
+        if (step2 && damage) mark_plane_due(a.pflag, nz, sid + 2u, k, due2);
         // plane p's buffer last held plane p-2, whose readers passed the
         // previous iteration's barrier
         sT1[p & 1][ty][tx] = t1p;

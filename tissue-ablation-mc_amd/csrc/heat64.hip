@@ -79,6 +79,12 @@ __global__ __launch_bounds__(256) void heat64_kernel(Heat64Args a, const double*
                 const double om = a.omega[v] + a.delt * a.arr_A * exp(-a.arr_dE / (a.arr_R * tn));
                 a.omega[v] = om;
                 thres_chain(a.thres, a.tmask, v, nv, om, a.thr0, a.thr1, a.thr2, time_after, a.tmask[v]);
+            } else {
+                // outside the window the reference's chain still runs: a
+                // threshold left due by an earlier step is written now
+                const unsigned m = a.tmask[v];
+                if (m & TAM_TMASK_DUE)
+                    thres_chain(a.thres, a.tmask, v, nv, a.omega[v], a.thr0, a.thr1, a.thr2, time_after, m);
             }
         }
         if (ENERGY && (i == 1 || i == a.nx || j == 1 || j == a.ny || k == 1 || k == a.nz)) {

@@ -288,13 +288,21 @@ hipError_t launch_energy(int nx, int ny, int nz, long px, long pxy, const float*
     return hipGetLastError();
 }
 namespace {
-__global__ __launch_bounds__(256) void tmask_kernel(const double* __restrict__ thres, uint8_t* __restrict__ tmask,
-                                                    long nvox)
+__global__ __launch_bounds__(256) void tmask_kernel(const double* __restrict__ thres,
+                                                    const double* __restrict__ omega, double thr0, double thr1,
+                                                    double thr2, uint8_t* __restrict__ tmask, long nvox)
 {
     const long v = (long)blockIdx.x * 256 + threadIdx.x;
     if (v >= nvox) return;
-    tmask[v] = (uint8_t)((thres[v] != 0.0 ? 1u : 0u) | (thres[v + nvox] != 0.0 ? 2u : 0u) |
-                         (thres[v + 2 * nvox] != 0.0 ? 4u : 0u));
+    const unsigned m = (thres[v] != 0.0 ? 1u : 0u) | (thres[v + nvox] != 0.0 ? 2u : 0u) |
+                       (thres[v + 2 * nvox] != 0.0 ? 4u : 0u);
+    const double om = omega[v];
+    const bool due = (!(m & 1u) && om >= thr0) || (!(m & 2u) && om >= thr1) || (!(m & 4u) && om >= thr2);
+    tmask[v] = (uint8_t)(m | (due ? TAM_TMASK_DUE : 0u));
+}
+__global__ void fill_u32_kernel(uint32_t* p, long n, uint32_t v)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
 }
 }  // namespace
 
@@ -326,10 +334,19 @@ hipError_t launch_class_flags(uint8_t* cls, int nx, int ny, int nz, long px, lon
     return hipGetLastError();
 }
 
-hipError_t launch_tmask(const double* thres, uint8_t* tmask, long nvox, hipStream_t s)
+hipError_t launch_tmask(const double* thres, const double* omega, double thr0, double thr1, double thr2,
+                        uint8_t* tmask, long nvox, hipStream_t s)
 {
     if (nvox <= 0) return hipSuccess;
-    hipLaunchKernelGGL(tmask_kernel, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, s, thres, tmask, nvox);
+    hipLaunchKernelGGL(tmask_kernel, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, s, thres, omega, thr0, thr1,
+                       thr2, tmask, nvox);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, long n, uint32_t v, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_fill(n)), dim3(256), 0, s, p, n, v);
     return hipGetLastError();
 }
 

@@ -22,28 +22,55 @@
 
 #define TAM_MAX_MATERIALS 255
 
-// First-crossing times of the damage thresholds (3dFD.f90:447-464): the
-// reference's chain "if (ThresTime(1) == 0 .and. Omega >= thr1) ... else if
-// (ThresTime(2) == 0 ...) ... else if (ThresTime(3) == 0 ...)", with each
-// "ThresTime(k) == 0" read from bit k of the voxel's mask m instead of from
-// the FP64 time, so a step reads one byte rather than up to three doubles.
-// A time written as 0.0 leaves its bit clear, as ThresTime(k) stays 0 then.
-// Returns the voxel's mask after the call.
+// First-crossing times of the damage thresholds (3dFD.f90:447-464): one call
+// of the reference's chain "if (ThresTime(1) == 0 .and. Omega >= thr1) ...
+// else if (ThresTime(2) == 0 ...) ... else if (ThresTime(3) == 0 ...)" at
+// time t, with each "ThresTime(k) == 0" read from bit k of the voxel's mask m
+// instead of from the FP64 time, so a step reads one byte rather than up to
+// three doubles.  A time written as 0.0 leaves its bit clear, as ThresTime(k)
+// stays 0 then.
+//
+// The reference runs the chain for every voxel on every call, inside the
+// Arrhenius window or not, and writes at most one time per call.  A voxel
+// whose Omega passed two thresholds in one step therefore gets the second
+// time on the next call whatever its temperature: after the call, bit
+// TAM_TMASK_DUE says that some threshold with an unset time is already
+// reached, so the step kernels run the chain for that voxel in the next step
+// even outside the window.  Returns the voxel's mask after the call (stored
+// when it changed).
+#define TAM_TMASK_DUE 8u
 __device__ __forceinline__ unsigned thres_chain(double* thres, uint8_t* tmask, long v, long nv, double om,
                                                 double thr0, double thr1, double thr2, double t, unsigned m)
 {
+    const unsigned m0 = m;
     int k = -1;
     if (!(m & 1u) && om >= thr0) k = 0;
     else if (!(m & 2u) && om >= thr1) k = 1;
     else if (!(m & 4u) && om >= thr2) k = 2;
     if (k >= 0) {
         thres[v + k * nv] = t;
-        if (t != 0.0) {
-            m |= 1u << k;
-            tmask[v] = (uint8_t)m;
-        }
+        if (t != 0.0) m |= 1u << k;
     }
+    const bool due = (!(m & 1u) && om >= thr0) || (!(m & 2u) && om >= thr1) || (!(m & 4u) && om >= thr2);
+    m = due ? (m | TAM_TMASK_DUE) : (m & ~TAM_TMASK_DUE);
+    if (m != m0) tmask[v] = (uint8_t)m;
     return m;
+}
+
+// Planes with a due chain: the step of id s reads slot s % 3 of the
+// 3 x (nz + 2) step-tagged plane flags and checks plane k when the flag holds
+// s; a launch whose last step is s' tags the planes where it left a chain due
+// with s' + 1 in slot (s' + 1) % 3.  One- and two-step launches thus never
+// read the slot they write, and a stale tag never matches a later step.
+__device__ __forceinline__ bool plane_due(const uint32_t* pflag, int nz, uint32_t sid, int k)
+{
+    return pflag[(size_t)(sid % 3u) * (size_t)(nz + 2) + (size_t)k] == sid;
+}
+__device__ __forceinline__ void mark_plane_due(uint32_t* pflag, int nz, uint32_t sid_next, int k, bool due)
+{
+    const uint64_t w = __ballot(due);
+    if (w && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)w) - 1))
+        pflag[(size_t)(sid_next % 3u) * (size_t)(nz + 2) + (size_t)k] = sid_next;
 }
 
 struct MatEntry {          // per material, in device memory
@@ -111,7 +138,9 @@ struct HeatArgs {
     // Arrhenius inside the step
     double* omega;             // interior
     double* thres;             // interior x3
-    uint8_t* tmask;            // interior: bit k set when thres[k] != 0
+    uint8_t* tmask;            // interior: bit k set when thres[k] != 0, TAM_TMASK_DUE
+    uint32_t* pflag;           // 3 x (nz+2) step-tagged planes with a due chain (see plane_due)
+    uint32_t sid;              // id of this launch's (first) step
     const double* rhokap;      // halo
     double arr_A, arr_dE, arr_R, arr_Tlo, arr_Thi, thr0, thr1, thr2;
     double lnA;                // log(arr_A), for the in-step damage rate
@@ -132,6 +161,7 @@ struct HeatArgs {
     const float* dyn_lon;
     const double* dyn_tafter;
     const double* dyn_arr;
+    const uint32_t* dyn_sid;   // graph replays: the cycle's first step id; step id = *dyn_sid + sid
 };
 
 // one instance of a batched heat step: its arguments and its ping-pong pair
@@ -240,10 +270,13 @@ struct tam_ctx {
     MatEntry* mtab;
     double* mua_frac;           // per material mu_a / mu_t (FP64)
     double *tally, *jmean, *Q, *W, *omega, *thres;
-    uint8_t* tmask;             // bit k: thres[k] != 0 (kept with every write of thres)
+    uint8_t* tmask;             // bit k: thres[k] != 0 (kept with every write of thres), TAM_TMASK_DUE
+    uint32_t* pflag;            // 3 x (nz+2) step-tagged planes with a due threshold chain
+    uint32_t sid;               // id of the next heat step (steps since creation)
     // heat tuner scratch copies (allocated on the first tam_tune_heat, kept)
     double *tn_omega, *tn_thres, *tn_Q, *tn_eblk;
     uint8_t* tn_tmask;
+    uint32_t* tn_pflag;
     double* box_psum;           // LDS tally box placement: per-plane sums (nz) and the chosen
     int* box_geom;              // geometry (o[3], n[3])
     double* energy;             // [0] boundary outflow, [1..] reduction scratch
@@ -276,7 +309,9 @@ struct CycleHdr {
     int32_t mc_on;             // 1: run transport + normalise
     double scale;              // normalisation scale
     double arr[2];             // Arrhenius delt, time (reference cadence)
-    double pad[3];
+    uint32_t sid0;             // id of the cycle's first heat step
+    uint32_t pad0;
+    double pad[2];
 };
 static_assert(sizeof(CycleHdr) == 64, "cycle header layout");
 
@@ -333,8 +368,11 @@ hipError_t launch_normalise(int nx, int ny, int nz, long px, long pxy, const dou
                             const CycleHdr* dyn, double scale, const double* coeff, double* jmean,
                             float* src, hipStream_t s);
 hipError_t launch_fill_f64(double* p, long n, double v, hipStream_t s);
-// tmask from the threshold times: bit k = (thres[k * nvox + v] != 0)
-hipError_t launch_tmask(const double* thres, uint8_t* tmask, long nvox, hipStream_t s);
+// tmask from the threshold times and Omega: bit k = (thres[k * nvox + v] != 0),
+// TAM_TMASK_DUE where a threshold with an unset time is already reached
+hipError_t launch_tmask(const double* thres, const double* omega, double thr0, double thr1, double thr2,
+                        uint8_t* tmask, long nvox, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, long n, uint32_t v, hipStream_t s);
 hipError_t launch_fill_f32(float* p, long n, float v, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, long n, hipStream_t s);
 hipError_t launch_copy(const void* in, void* out, long bytes, hipStream_t s);
