@@ -309,6 +309,27 @@ int  tam_batch_tune(tam_batch* b, const int64_t* sample_packets, int32_t heat_re
 /* device time of the last batched launch sequence, ms */
 double tam_batch_last_ms(const tam_batch* b);
 
+/* --- spectral sampling helpers (ch_opt.f90:28-152; compiled but never
+ * called in the reference, which runs one wavelength).  A spectrum is the
+ * reference's table array(len, 2), column-major: table[i] the wavelength of
+ * row i+1 (strictly ascending), table[len + i] its value.  No context:
+ * errors are read with tam_last_error(NULL).
+ * tam_spectrum_cdf replaces mk_cdf (ch_opt.f90:138-152): cdf[j-1] = the
+ *   trapezoid area of rows 1..j+1 over the whole area, j = 1..len-1;
+ *   cdf[len-1] is divided like the rest (the reference leaves that entry to
+ *   its caller).  Host arithmetic, the reference's operation order.
+ * tam_spectrum_sample replaces n calls of sample (ch_opt.f90:28-45, with
+ *   search_1D :97-116 and lin_inter_1D :63-76) on `device`: draw i takes
+ *   its uniform from Philox4x32-10 (key = seed, counter = (i, 0, stream),
+ *   first word, ((x >> 9) + 0.5) * 2^-23) in place of ran2, and is inverted
+ *   through the CDF in FP64 as the reference does (len >= 3).
+ * tam_spectrum_interp replaces search_2D + lin_inter_2D (ch_opt.f90:79-135):
+ *   the table's value at each of n wavelengths, on `device`. */
+int tam_spectrum_cdf(const double* table, int32_t len, double* cdf);
+int tam_spectrum_sample(int32_t device, const double* table, const double* cdf, int32_t len, int64_t n,
+                        uint64_t seed, uint32_t stream, double* wave);
+int tam_spectrum_interp(int32_t device, const double* table, int32_t len, const double* x, int64_t n, double* y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,8 @@
 !                     driven like the photon loop of mcpolar.f90:151-170
 !   ref_stokes     -> stokes.f90:6-153
 !   ref_fits       -> thermalConst_mod.f90:11-87
+!   ref_mk_cdf, ref_sample, ref_search, ref_lin_inter
+!                  -> ch_opt.f90:28-152 (spectral helpers)
 ! The reference grid is fixed at compile time (constants.f90:12, 80^3).
 module ref_harness
     use iso_c_binding
@@ -120,5 +122,80 @@ contains
             out(6, i) = getSkinThermalCond(ww, rr)
         end do
     end subroutine ref_fits
+
+    ! mk_cdf on a (n, 2) table; cdf(n) comes in as the caller's value
+    subroutine ref_mk_cdf(n, arr, cdf) bind(C, name="ref_mk_cdf")
+        use ch_opt, only : mk_cdf
+        integer(c_int), value         :: n
+        real(c_double), intent(in)    :: arr(n, 2)
+        real(c_double), intent(inout) :: cdf(n)
+        real :: a(n, 2), c(n)
+        a = arr
+        c = cdf
+        call mk_cdf(a, c, n)
+        cdf = c
+    end subroutine ref_mk_cdf
+
+    ! ns successive sample calls (one ran2 draw each) from seed iseed_c
+    subroutine ref_sample(n, ns, arr, cdf, iseed_c, waves) bind(C, name="ref_sample")
+        use ch_opt, only : sample
+        integer(c_int), value         :: n, ns
+        real(c_double), intent(in)    :: arr(n, 2), cdf(n)
+        integer(c_int), intent(inout) :: iseed_c
+        real(c_double), intent(out)   :: waves(ns)
+        real    :: a(n, 2), c(n), w
+        integer :: i, iseed
+        a = arr
+        c = cdf
+        iseed = iseed_c
+        do i = 1, ns
+            call sample(a, n, c, w, iseed)
+            waves(i) = w
+        end do
+        iseed_c = iseed
+    end subroutine ref_sample
+
+    ! nlow(i, 1) = search_1D over cdf, nlow(i, 2) = search_2D over arr's
+    ! first column, for each value(i)
+    subroutine ref_search(n, arr, cdf, nv, values, nlow) bind(C, name="ref_search")
+        use ch_opt, only : search_1D, search_2D
+        integer(c_int), value        :: n, nv
+        real(c_double), intent(in)   :: arr(n, 2), cdf(n), values(nv)
+        integer(c_int), intent(out)  :: nlow(nv, 2)
+        real    :: a(n, 2), c(n), v
+        integer :: i, k
+        a = arr
+        c = cdf
+        do i = 1, nv
+            v = values(i)
+            call search_1D(n, c, k, v)
+            nlow(i, 1) = k
+            call search_2D(n, a, k, v)
+            nlow(i, 2) = k
+        end do
+    end subroutine ref_search
+
+    ! out(i, 1) = lin_inter_1D(arr, cdf, values(i), n, nlow1(i)),
+    ! out(i, 2) = lin_inter_2D(arr, values(i), n, nlow2(i))
+    subroutine ref_lin_inter(n, arr, cdf, nv, values, nlow1, nlow2, out) bind(C, name="ref_lin_inter")
+        use ch_opt, only : lin_inter_1D, lin_inter_2D
+        integer(c_int), value       :: n, nv
+        real(c_double), intent(in)  :: arr(n, 2), cdf(n), values(nv)
+        integer(c_int), intent(in)  :: nlow1(nv), nlow2(nv)
+        real(c_double), intent(out) :: out(nv, 2)
+        real    :: a(n, 2), c(n), v, y
+        integer :: i, k
+        a = arr
+        c = cdf
+        do i = 1, nv
+            v = values(i)
+            k = nlow1(i)
+            call lin_inter_1D(a, c, v, n, k, y)
+            out(i, 1) = y
+            k = nlow2(i)
+            call lin_inter_2D(a, v, n, k, y)
+            out(i, 2) = y
+        end do
+    end subroutine ref_lin_inter
 
 end module ref_harness

@@ -896,3 +896,71 @@ void or_init_thermal(int n, double xmax, double ymax, double zmax, int nxg, int 
     o->massVoxel = o->density * o->volumeVoxel;
     o->QVapor = 2256.e3 * o->massVoxel;
 }
+
+/* ========================================================= spectral === */
+
+/* mk_cdf (ch_opt.f90:138-152): cdf(j) = sum_{i<=j} trapezoid(i, i+1) for
+ * j = 1..length-1, each sum restarted from zero and added in order; then
+ * every entry, cdf(length) included (left as the caller passed it), divided
+ * by cdf(length-1). */
+void or_mk_cdf(const double* array, double* cdf, int length)
+{
+    const double* x = array;
+    const double* y = array + length;
+    for (int j = 1; j <= length - 1; ++j) {
+        double summ = 0.0;
+        for (int i = 1; i <= j; ++i)
+            summ = summ + 0.5 * (y[i] + y[i - 1]) * (x[i] - x[i - 1]);
+        cdf[j - 1] = summ;
+    }
+    const double norm = cdf[length - 2];
+    for (int j = 0; j < length; ++j) cdf[j] = cdf[j] / norm;
+}
+
+/* search_1D (ch_opt.f90:97-116): bisection, value > array(middle) moves the
+ * lower end; array(length) itself is never compared. */
+int or_search_1d(int length, const double* array, double value)
+{
+    int nup = length, nlow = 1;
+    while (nup - nlow > 1) {
+        const int middle = (int)((double)(nup + nlow) / 2.0);
+        if (value > array[middle - 1]) nlow = middle;
+        else nup = middle;
+    }
+    return nlow;
+}
+
+/* search_2D (ch_opt.f90:119-135): the same on the first column. */
+int or_search_2d(int length, const double* array, double value)
+{
+    return or_search_1d(length, array, value);
+}
+
+/* lin_inter_1D (ch_opt.f90:63-76): cdf(j) is the CDF at row j+1, so the
+ * bracket [cdf(nlow), cdf(nlow+1)] maps onto rows nlow+1 .. nlow+2. */
+double or_lin_inter_1d(const double* array, const double* cdf, double value, int length, int nlow)
+{
+    (void)length;
+    const double* x = array;
+    return x[nlow] + (x[nlow + 1] - x[nlow]) * (value - cdf[nlow - 1]) / (cdf[nlow] - cdf[nlow - 1]);
+}
+
+/* lin_inter_2D (ch_opt.f90:79-94): the second column at `value` of the first. */
+double or_lin_inter_2d(const double* array, double value, int length, int nlow)
+{
+    const double* x = array;
+    const double* y = array + length;
+    return y[nlow - 1] + (y[nlow] - y[nlow - 1]) * (value - x[nlow - 1]) / (x[nlow] - x[nlow - 1]);
+}
+
+double or_sample_u(const double* array, int size_of, const double* cdf, double value)
+{
+    const int nlow = or_search_1d(size_of, cdf, value);
+    return or_lin_inter_1d(array, cdf, value, size_of, nlow);
+}
+
+/* sample (ch_opt.f90:28-45): one ran2 draw, inverted through the CDF. */
+double or_sample(const double* array, int size_of, const double* cdf, or_ran2_t* rng)
+{
+    return or_sample_u(array, size_of, cdf, or_ran2(rng));
+}

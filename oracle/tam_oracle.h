@@ -31,6 +31,10 @@
  *     balance of an insulated box with random conductivity.
  *   - or_arrhenius by the constant-temperature closed form; or_rebuild by a
  *     hand-computed crater and by brute force over tiny grids.
+ *   - the spectral helpers (or_mk_cdf .. or_sample) are pinned bit-for-bit
+ *     against the reference's ch_opt module compiled from its sources, and by
+ *     closed forms (a uniform spectrum samples x1 + u (xK - x1); a linear
+ *     density samples its nodes at u = CDF(node)).
  */
 #ifndef TAM_ORACLE_H
 #define TAM_ORACLE_H
@@ -201,6 +205,22 @@ void or_rebuild(or_rebuild_t* r);
  * mask (interior).  Used by tests to show the closed form equals the sweep. */
 void or_removal_mask(int nx, int ny, int nz, const double* temp, const double* rhokap,
                      double ablate_K, uint8_t* mask);
+
+/* ------------------------------------------------- spectral helpers --- */
+
+/* ch_opt.f90:28-152 (compiled in the reference, never called there): a
+ * tabulated spectrum array(len, 2), column-major -- array[i] the wavelength
+ * of row i+1, array[len + i] its value -- its trapezoid CDF, and sampling by
+ * inverse interpolation.  Indices are the reference's 1-based ones. */
+void   or_mk_cdf(const double* array, double* cdf, int length);                 /* :138-152 */
+int    or_search_1d(int length, const double* array, double value);             /* :97-116 */
+int    or_search_2d(int length, const double* array, double value);             /* :119-135 */
+double or_lin_inter_1d(const double* array, const double* cdf, double value, int length,
+                       int nlow);                                               /* :63-76 */
+double or_lin_inter_2d(const double* array, double value, int length, int nlow);   /* :79-94 */
+double or_sample(const double* array, int size_of, const double* cdf, or_ran2_t* rng);   /* :28-45 */
+/* sample from a given uniform (the draw the GPU takes from Philox) */
+double or_sample_u(const double* array, int size_of, const double* cdf, double value);
 
 /* ---------------------------------------- thermal fits, pulse, setup --- */
 

@@ -100,6 +100,7 @@ def lib():
         L = C.CDLL(ORACLE_SO)
         L.or_ran2.restype = C.c_double
         L.or_u01.restype = C.c_double
+        L.or_u01.argtypes = [C.c_uint32]
         L.or_fresnel.restype = C.c_double
         L.or_fresnel.argtypes = [C.c_double, C.c_double, C.c_double]
         L.or_ref_mc.restype = C.c_long
@@ -124,6 +125,17 @@ def lib():
         L.or_init_thermal.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                       C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(ThermalInit)]
         L.or_stokes.argtypes = [_dp, _dp, C.c_double, C.POINTER(Ran2)]
+        L.or_mk_cdf.argtypes = [_dp, _dp, C.c_int]
+        L.or_search_1d.argtypes = [C.c_int, _dp, C.c_double]
+        L.or_search_2d.argtypes = [C.c_int, _dp, C.c_double]
+        L.or_lin_inter_1d.argtypes = [_dp, _dp, C.c_double, C.c_int, C.c_int]
+        L.or_lin_inter_1d.restype = C.c_double
+        L.or_lin_inter_2d.argtypes = [_dp, C.c_double, C.c_int, C.c_int]
+        L.or_lin_inter_2d.restype = C.c_double
+        L.or_sample.argtypes = [_dp, C.c_int, _dp, C.POINTER(Ran2)]
+        L.or_sample.restype = C.c_double
+        L.or_sample_u.argtypes = [_dp, C.c_int, _dp, C.c_double]
+        L.or_sample_u.restype = C.c_double
         _lib = L
     return _lib
 
@@ -146,6 +158,11 @@ def ref():
                              C.c_int, _dp, _dp, C.POINTER(C.c_long)]
         R.ref_stokes.argtypes = [C.c_int, C.c_double, C.POINTER(C.c_int), _dp, _dp]
         R.ref_fits.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, _dp, _dp]
+        _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        R.ref_mk_cdf.argtypes = [C.c_int, _dp, _dp]
+        R.ref_sample.argtypes = [C.c_int, C.c_int, _dp, _dp, C.POINTER(C.c_int), _dp]
+        R.ref_search.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _ip]
+        R.ref_lin_inter.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _ip, _ip, _dp]
         _ref = R
     return _ref
 
@@ -217,3 +234,25 @@ class MCProblem:
         rc = lib().or_mc_run(C.byref(self.m), p0, npk, tally, C.byref(st), nthreads)
         assert rc == 0
         return tally.reshape(self.shape), st
+
+
+# ------------------------------------------------------ spectral helpers --
+
+def spectrum_table(x, y):
+    """(len, 2) column-major table as the reference stores array(len, 2)."""
+    return np.ascontiguousarray(np.concatenate([np.asarray(x, float), np.asarray(y, float)]))
+
+
+def mk_cdf(table, cdf_last=0.0):
+    n = table.size // 2
+    cdf = np.zeros(n)
+    cdf[-1] = cdf_last
+    lib().or_mk_cdf(table, cdf, n)
+    return cdf
+
+
+def sample_seq(table, cdf, seed, ns):
+    L = lib()
+    s = Ran2()
+    L.or_ran2_init(C.byref(s), C.c_int32(seed))
+    return np.array([L.or_sample(table, table.size // 2, cdf, C.byref(s)) for _ in range(ns)])

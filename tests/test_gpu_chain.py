@@ -211,7 +211,11 @@ def test_reference_mode_chain_and_writer_fields():
             tally = g.download(lib.FIELD_TALLY)
             t_or, ost = _oracle_mc(prob, g, driver.SEED, cyc)
             _check_transport(prob, st, tally, ost, t_or)
-            scale = 1e-3 * (power(pulse) / driver.SPOTS) / (prob.packets * p.volume_voxel)
+            # 3e-3 of the input's 70 W on the whole grid: the top planes
+            # reach the boiling plateau and the damage thresholds within the
+            # pulse without being ablated (the full power vaporises them in a
+            # few steps)
+            scale = 3e-3 * power(pulse) / (prob.packets * p.volume_voxel)
             g.normalise(scale)
             jm = _check_normalise(prob, g, tally, scale)
         else:

@@ -37,9 +37,7 @@ def _crater_skin(n=40, packets=200_000):
     rk[1:-1, 1:-1, 1:-1] = mut[prob.mat]
     zz, yy, xx = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
     crater = ((xx - n / 2 + 0.5) ** 2 + (yy - n / 2 + 0.5) ** 2 <= 25) & (zz >= n - 6)
-    rk[1:-1, 1:-1, 1:-1][crater] = 0.0
-    prob.mat = prob.mat.copy()
-    prob.mat[crater] = len(prob.table)                 # the air entry tam_set_optics appends
+    rk[1:-1, 1:-1, 1:-1][crater] = 0.0                 # ablated: mu_t = 0, the material id kept
     prob.rhokap = rk
     return prob
 
@@ -61,7 +59,7 @@ def test_normalise_is_tally_times_mua_times_scale():
     frac = np.array([t[0] / (t[0] + t[1]) for t in prob.table] + [0.0])
     expect = tally * (rk * frac[mat]) * scale
     assert np.array_equal(jm, expect)
-    assert not jm[mat == len(prob.table)].any() and jm.max() > 0
+    assert (rk == 0).any() and not jm[rk == 0].any() and jm.max() > 0
 
 
 def test_track_length_estimator_matches_absorbed_packets():

@@ -10,7 +10,8 @@ numpy, through the C-ABI:
     sums of the downloaded tally (1e-12 relative); tam_dn_apply against
     tam_dn_forward_host on the same features (the prediction, back as path
     length); and the residual fallback: with tolerance 0 every cycle takes the
-    high-count branch, so the run equals the plain form exactly.
+    high-count branch, so the run follows the plain form (1e-6 relative in T,
+    the same crater).
 """
 import ctypes as C
 import json
@@ -165,8 +166,10 @@ def test_dn_apply_matches_forward_on_host_features():
 def test_residual_fallback_takes_the_high_count_branch():
     """Tolerance 0: every cycle's residual exceeds it, so each cycle runs the
     high-count transport on the same stream as the plain form; the final
-    temperature field equals the plain run's bit for bit (same kernels, same
-    launch shapes), and every cycle is counted as a fallback."""
+    temperature field equals the plain run's (same kernels, launch shapes and
+    packet histories; the FP64 tally atomics may sum in another order, and
+    the LDS tally box is placed from the last run's path, which differs, so
+    to 1e-6 relative), and every cycle is counted as a fallback."""
     prob = _skin64(100_000)
     dn = Denoiser((64, 64, 64), capacity=0)
     dn.set_params(np.fromfile(WEIGHTS, dtype=np.float32))
@@ -177,7 +180,8 @@ def test_residual_fallback_takes_the_high_count_branch():
     dn.close()
     assert forced["fallbacks"] == 2 and all(c.get("fallback") for c in forced["cycles"])
     assert [c["npk"] for c in forced["cycles"]] == [100_000, 100_000]
-    assert np.array_equal(forced["T"], plain["T"])
+    np.testing.assert_allclose(forced["T"], plain["T"], rtol=1e-6, atol=0)
+    assert np.array_equal(forced["mat"], plain["mat"])
     assert loose["fallbacks"] == 0 and [c["npk"] for c in loose["cycles"]] == [10_000, 10_000]
     rep = coupled.compare(prob, plain, loose)
     _print("loose vs plain", {k: v for k, v in rep.items() if not isinstance(v, list)})

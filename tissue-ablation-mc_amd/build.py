@@ -11,7 +11,7 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
-SOURCES = ["mc.hip", "heat.hip", "heat64.hip", "rebuild.hip", "api.hip", "denoise.hip"]
+SOURCES = ["mc.hip", "heat.hip", "heat64.hip", "rebuild.hip", "api.hip", "denoise.hip", "spectral.hip"]
 LIB = os.path.join(PKG, "libtam.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")

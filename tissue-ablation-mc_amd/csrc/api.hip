@@ -27,6 +27,12 @@ int fail(tam_ctx* c, int code, const std::string& msg)
     return code;
 }
 
+}  // namespace
+
+int tam_fail_noctx(int code, const std::string& msg) { return fail(nullptr, code, msg); }
+
+namespace {
+
 int hip_fail(tam_ctx* c, const char* what, hipError_t e)
 {
     return fail(c, TAM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));

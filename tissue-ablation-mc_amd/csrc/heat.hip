@@ -211,6 +211,7 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
     long v = (long)(ic - 1) + (long)nx * (jc - 1) + nxy * (k0 - 1);
     const float wlo = a.dT_lo_test - a.dT_win_pad, whi = a.dT_hi_test + a.dT_win_pad;
     const uint32_t sid = step_id(a);
+    const uint64_t dmask = damage ? due_planes(a.pflag, nz, sid, k0, k1) : 0ull;
     bool pom = damage && in && tc >= wlo && tc < whi;
     double om_pf = 0.0;
     unsigned tm_pf = 0;
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
             }
         }
         __syncthreads();
-        const bool chk = damage && plane_due(a.pflag, nz, sid, k);
+        const bool chk = (dmask >> (k - k0)) & 1ull;
         if (!in) continue;
         // ---- update of voxel (i, j, k)
         const float txm = sT[ty + 1][tx], txp = sT[ty + 1][tx + 2];
@@ -397,6 +398,7 @@ __device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __re
     }
     float e_acc = 0.f;
     const uint32_t sid = step_id(a);
+    const uint64_t dmask = damage ? due_planes(a.pflag, nz, sid, k0, k1) : 0ull;
     for (int k = k0; k < k1; ++k, c += pxy, v += nxy) {
         const float tp0 = Tin[c + pxy];
         const float txm = Tin[c - 1], txp = Tin[c + 1], tym = Tin[c - px], typ = Tin[c + px];
@@ -460,7 +462,7 @@ __device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __re
             }
             Tout[c] = tn;
             if (damage)                                                  // 3dFD.f90:450-461
-                due = damage_one(a, v, nxy * nz, tn, t_after, plane_due(a.pflag, nz, sid, k), pom, om_pf, tm_pf);
+                due = damage_one(a, v, nxy * nz, tn, t_after, (dmask >> (k - k0)) & 1ull, pom, om_pf, tm_pf);
             if (energy && (bnd_ij || k == 1 || k == nz)) {
                 float e = 0.f;
                 if (i == 1) e += gxm * (txm - tc);
@@ -607,6 +609,7 @@ __device__ __forceinline__ void t2_body(const HeatArgs& a, const float* __restri
 
     const long nxy = (long)nx * ny, nv = nxy * nz;
     const uint32_t sid = step_id(a);                             // id of the launch's first step
+    const uint64_t dmask = damage ? due_planes(a.pflag, nz, sid, k0, k1) : 0ull;
     const long vcol = (long)(i - 1) + (long)nx * (j - 1);        // interior index of the column (own lanes)
     const float wlo = a.dT_lo_test - a.dT_win_pad, whi = a.dT_hi_test + a.dT_win_pad;
     // first step at plane p: dT0 at p-1, p (z0m, z0c), classes at p-1, p
@@ -699,7 +702,7 @@ __device__ __forceinline__ void t2_body(const HeatArgs& a, const float* __restri
             }
         }
         // ---- second step at plane k = p-1 (dT1 of plane k is in sT1[k & 1])
-        const bool chk = step2 && damage && plane_due(a.pflag, nz, sid, k);
+        const bool chk = step2 && ((dmask >> (k - k0)) & 1ull);
         bool due2 = false;
         if (step2 && own) {
             const float xm = sT1[k & 1][ty][tx - 1], xp = sT1[k & 1][ty][tx + 1];
@@ -970,6 +973,7 @@ hipError_t launch_heat_step2(const HeatArgs& a, const float* Tin, float* Tout, f
 {
     const unsigned f = (robin ? F_ROBIN : 0u) | (damage ? F_DAMAGE : 0u) | (energy ? F_ENERGY : 0u);
     if (kz < 1) kz = 1;
+    if (kz > TAM_MAX_KZ) kz = TAM_MAX_KZ;
     if (kz > a.nz) kz = a.nz;
     if (!a.cls || !a.glut) return hipErrorInvalidValue;      // class mode only
     switch (tile) {
@@ -987,6 +991,7 @@ hipError_t launch_heat_step(const HeatArgs& a, const float* Tin, float* Tout, fl
     const unsigned f = (boil ? F_BOIL : 0u) | (robin ? F_ROBIN : 0u) | (damage ? F_DAMAGE : 0u) |
                        (energy ? F_ENERGY : 0u);
     if (kz < 1) kz = 1;
+    if (kz > TAM_MAX_KZ) kz = TAM_MAX_KZ;
     if (kz > a.nz) kz = a.nz;
     if (tile == 6 || tile == 7) tile = 4;       // two-step shapes: one step on direct-32x8 / direct-64x8
     else if (tile == 8) tile = 5;

@@ -49,26 +49,6 @@
 
 namespace {
 
-__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1)
-{
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;     // one v_mad_u64_u32 each
-        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
-        c = make_uint4((uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1,
-                       (uint32_t)p0);
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-    return c;
-}
-
-// 23-bit uniform on (0,1), exact in FP32: ((x >> 9) + 0.5) * 2^-23.
-__device__ __forceinline__ float u01(uint32_t x)
-{
-    return ((float)(x >> 9) + 0.5f) * (1.0f / 8388608.0f);
-}
-
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float neg_ln(float u) { return -0.69314718f * __builtin_amdgcn_logf(u); }

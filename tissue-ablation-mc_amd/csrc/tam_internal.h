@@ -22,6 +22,27 @@
 
 #define TAM_MAX_MATERIALS 255
 
+// Philox4x32-10 (Salmon et al., SC'11): counter c, key (k0, k1).
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;     // one v_mad_u64_u32 each
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        c = make_uint4((uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1,
+                       (uint32_t)p0);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// 23-bit uniform on (0,1), exact in FP32: ((x >> 9) + 0.5) * 2^-23.
+__device__ __forceinline__ float u01(uint32_t x)
+{
+    return ((float)(x >> 9) + 0.5f) * (1.0f / 8388608.0f);
+}
+
 // First-crossing times of the damage thresholds (3dFD.f90:447-464): one call
 // of the reference's chain "if (ThresTime(1) == 0 .and. Omega >= thr1) ...
 // else if (ThresTime(2) == 0 ...) ... else if (ThresTime(3) == 0 ...)" at
@@ -62,9 +83,13 @@ __device__ __forceinline__ unsigned thres_chain(double* thres, uint8_t* tmask, l
 // s; a launch whose last step is s' tags the planes where it left a chain due
 // with s' + 1 in slot (s' + 1) % 3.  One- and two-step launches thus never
 // read the slot they write, and a stale tag never matches a later step.
-__device__ __forceinline__ bool plane_due(const uint32_t* pflag, int nz, uint32_t sid, int k)
+// A block reads the flags of its planes k0 .. k1-1 (at most 64) once, one
+// plane per lane, into a wave-uniform mask: bit k - k0 says plane k is due.
+__device__ __forceinline__ uint64_t due_planes(const uint32_t* pflag, int nz, uint32_t sid, int k0, int k1)
 {
-    return pflag[(size_t)(sid % 3u) * (size_t)(nz + 2) + (size_t)k] == sid;
+    const int t = (int)(threadIdx.x & 63u);
+    const bool f = t < k1 - k0 && pflag[(size_t)(sid % 3u) * (size_t)(nz + 2) + (size_t)(k0 + t)] == sid;
+    return __ballot(f);
 }
 __device__ __forceinline__ void mark_plane_due(uint32_t* pflag, int nz, uint32_t sid_next, int k, bool due)
 {
@@ -72,6 +97,8 @@ __device__ __forceinline__ void mark_plane_due(uint32_t* pflag, int nz, uint32_t
     if (w && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)w) - 1))
         pflag[(size_t)(sid_next % 3u) * (size_t)(nz + 2) + (size_t)k] = sid_next;
 }
+// the launch shapes' largest plane count per block (due_planes reads one plane per lane)
+constexpr int TAM_MAX_KZ = 64;
 
 struct MatEntry {          // per material, in device memory
     float albedo, g, n, mua;
@@ -368,6 +395,8 @@ hipError_t launch_normalise(int nx, int ny, int nz, long px, long pxy, const dou
                             const CycleHdr* dyn, double scale, const double* coeff, double* jmean,
                             float* src, hipStream_t s);
 hipError_t launch_fill_f64(double* p, long n, double v, hipStream_t s);
+// error of a call that has no context (read back by tam_last_error(NULL))
+int tam_fail_noctx(int code, const std::string& msg);
 // tmask from the threshold times and Omega: bit k = (thres[k * nvox + v] != 0),
 // TAM_TMASK_DUE where a threshold with an unset time is already reached
 hipError_t launch_tmask(const double* thres, const double* omega, double thr0, double thr1, double thr2,

@@ -173,6 +173,32 @@ module tam_bind
             type(c_ptr), value :: ctx
         end function
 
+        ! spectral helpers (ch_opt.f90:28-152): table is the reference's
+        ! array(len, 2); errors through tam_message(c_null_ptr)
+        integer(c_int) function tam_spectrum_cdf(table, len, cdf) bind(C, name="tam_spectrum_cdf")
+            import :: c_int, c_int32_t, c_double
+            real(c_double), intent(in)    :: table(*)
+            integer(c_int32_t), value     :: len
+            real(c_double), intent(inout) :: cdf(*)
+        end function
+
+        integer(c_int) function tam_spectrum_sample(device, table, cdf, len, n, seed, stream, wave) &
+                bind(C, name="tam_spectrum_sample")
+            import :: c_int, c_int32_t, c_int64_t, c_double
+            integer(c_int32_t), value   :: device, len, stream
+            real(c_double), intent(in)  :: table(*), cdf(*)
+            integer(c_int64_t), value   :: n, seed
+            real(c_double), intent(out) :: wave(*)
+        end function
+
+        integer(c_int) function tam_spectrum_interp(device, table, len, x, n, y) bind(C, name="tam_spectrum_interp")
+            import :: c_int, c_int32_t, c_int64_t, c_double
+            integer(c_int32_t), value   :: device, len
+            real(c_double), intent(in)  :: table(*), x(*)
+            integer(c_int64_t), value   :: n
+            real(c_double), intent(out) :: y(*)
+        end function
+
         integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
             import :: c_size_t, c_ptr
             type(c_ptr), value :: s

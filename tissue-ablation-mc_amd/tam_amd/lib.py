@@ -30,7 +30,8 @@ EXPORTED = ("tam_abi_version", "tam_create", "tam_destroy", "tam_last_error", "t
             "tam_set_mc_box", "tam_get_mc_box",
             "tam_copy_bandwidth", "tam_atomic_rate", "tam_struct_size", "tam_pulse_advance", "tam_cycle",
             "tam_tally_blocks", "tam_batch_create", "tam_batch_destroy", "tam_batch_last_error",
-            "tam_batch_mc_run", "tam_batch_heat_run", "tam_batch_set_launch", "tam_batch_tune", "tam_batch_last_ms")
+            "tam_batch_mc_run", "tam_batch_heat_run", "tam_batch_set_launch", "tam_batch_tune", "tam_batch_last_ms",
+            "tam_spectrum_cdf", "tam_spectrum_sample", "tam_spectrum_interp")
 
 
 class Params(C.Structure):
@@ -152,6 +153,9 @@ def load():
                             C.POINTER(Pulse), C.POINTER(MCStats)]
     L.tam_tune_heat.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                 C.POINTER(C.c_double)]
+    L.tam_spectrum_cdf.argtypes = [dp, C.c_int32, dp]
+    L.tam_spectrum_sample.argtypes = [C.c_int32, dp, dp, C.c_int32, C.c_int64, C.c_uint64, C.c_uint32, dp]
+    L.tam_spectrum_interp.argtypes = [C.c_int32, dp, C.c_int32, dp, C.c_int64, dp]
     _lib = L
     return L
 
