@@ -261,8 +261,9 @@ __global__ __launch_bounds__(256) void block_sum_kernel(const double* f, int nx,
     const long b = blockIdx.x;
     const int ib = (int)(b % bx), jb = (int)((b / bx) % by), kb = (int)(b / ((long)bx * by));
     double s = 0.0;
-    for (int q = threadIdx.x; q < k * k * k; q += 256) {
-        const int i = q % k, j = (q / k) % k, l = q / (k * k);
+    const long kkk = (long)k * k * k;                  // k may be a whole 4096-voxel edge
+    for (long q = threadIdx.x; q < kkk; q += 256) {
+        const int i = (int)(q % k), j = (int)((q / k) % k), l = (int)(q / ((long)k * k));
         s += f[(long)(ib * k + i) + (long)nx * ((jb * k + j) + (long)ny * (kb * k + l))];
     }
     red[threadIdx.x] = s;
