@@ -31,6 +31,7 @@ int fail(tam_ctx* c, int code, const std::string& msg)
 
 int tam_fail_noctx(int code, const std::string& msg) { return fail(nullptr, code, msg); }
 
+
 namespace {
 
 int hip_fail(tam_ctx* c, const char* what, hipError_t e)
@@ -103,7 +104,6 @@ HeatArgs heat_args(tam_ctx* c)
     a.dT_boil = (float)(373.0 - c->p.t_ref);
     a.t_ref = c->p.t_ref;
     a.omega = c->omega; a.thres = c->thres; a.tmask = c->tmask; a.rhokap = c->rhokap;
-    a.pflag = c->pflag; a.sid = c->sid;
     a.arr_A = c->p.arr_A; a.arr_dE = c->p.arr_dE; a.arr_R = c->p.arr_R;
     a.arr_Tlo = c->p.arr_Tlo; a.arr_Thi = c->p.arr_Thi;
     a.thr0 = c->p.arr_thr[0]; a.thr1 = c->p.arr_thr[1]; a.thr2 = c->p.arr_thr[2];
@@ -184,6 +184,7 @@ int classify(tam_ctx* c, const double* K, const double* D, const double* Hc, con
     }
     HIPCHK(c, hipMemcpyAsync(c->cls, cls.data(), (size_t)H, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, launch_class_flags(c->cls, c->p.nx, c->p.ny, c->p.nz, c->px, c->pxy, c->stream));
+    HIPCHK(c, launch_class_due(c->cls, c->tmask, c->p.nx, c->p.ny, c->p.nz, c->px, c->pxy, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->glut, lut.data(), sizeof(float) * lut.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->robin_lut, rob.data(), sizeof(float2) * rob.size(), hipMemcpyHostToDevice,
                              c->stream));
@@ -371,8 +372,6 @@ int enqueue_cycle(tam_ctx* c, int* swaps)
         } else {
             a.dyn_lon = lon + k;
             a.dyn_tafter = ta + k;
-            a.dyn_sid = &hdr->sid0;                 // step id = the cycle's first + k
-            a.sid = (uint32_t)k;
             if (two && k + 1 < c->cyc_loops) {      // steps k and k+1 in one launch
                 HIPCHK(c, launch_heat_step2(a, c->cur, c->nxt, 0.f, 0.f, 0.0, 0.0, robin, dmg, 1, c->tile, c->kz,
                                             s));
@@ -536,7 +535,6 @@ int tam_create(const tam_params* p, tam_ctx** out)
     if (e == hipSuccess) e = dalloc(&c->omega, V);
     if (e == hipSuccess) e = dalloc(&c->thres, 3 * V);
     if (e == hipSuccess) e = dalloc(&c->tmask, V);
-    if (e == hipSuccess) e = dalloc(&c->pflag, 3 * (nz + 2));
     if (e == hipSuccess) e = dalloc(&c->box_psum, nz);
     if (e == hipSuccess) e = dalloc(&c->box_geom, 6);
     if (e == hipSuccess) e = hipMemsetAsync(c->box_geom, 0, 6 * sizeof(int), c->stream);
@@ -571,8 +569,6 @@ int tam_create(const tam_params* p, tam_ctx** out)
     launch_fill_f64(c->omega, V, 0.0, s);
     launch_fill_f64(c->thres, 3 * V, 0.0, s);
     launch_tmask(c->thres, c->omega, p->arr_thr[0], p->arr_thr[1], p->arr_thr[2], c->tmask, V, s);
-    launch_fill_u32(c->pflag, 3 * (nz + 2), 0xffffffffu, s);     // no plane has a due chain
-    c->sid = 0;
     launch_fill_f64(c->energy, 4, 0.0, s);
     launch_fill_f64(c->eblk, c->neblk, 0.0, s);
     launch_fill_f32((float*)c->flags, 4, 0.f, s);
@@ -601,7 +597,7 @@ void tam_destroy(tam_ctx* c)
     if (c->stream) hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->dT0, c->dT1, c->gx, c->gy, c->gz, c->src, c->robin, c->kappa, c->density,
                     c->heatcap, c->coeff, c->rhokap, c->mut, c->mat, c->mask, c->mtab, c->mua_frac,
-                    c->tally, c->jmean, c->Q, c->W, c->omega, c->thres, c->tmask, c->pflag, c->tn_pflag,
+                    c->tally, c->jmean, c->Q, c->W, c->omega, c->thres, c->tmask, c->tn_cls,
                     c->energy, c->flags,
                     c->counters, c->T64a, c->T64b, c->ax, c->ay, c->az, c->cls, c->glut,
                     c->robin_lut, c->eblk, c->box_psum, c->box_geom, c->tn_omega, c->tn_thres, c->tn_Q,
@@ -808,7 +804,7 @@ int tam_heat_run(tam_ctx* c, int32_t loops, tam_pulse* pulse)
     if (loops < 0) return fail(c, TAM_ERR_ARG, "negative loop count");
     if (!c->thermal_set) return fail(c, TAM_ERR_STATE, "tam_set_thermal must come first");
     Dev d(c->dev);
-    HeatArgs a = heat_args(c);
+    const HeatArgs a = heat_args(c);
     const Heat64Args a64 = heat64_args(c);
     const int boil = c->p.boil != 0, robin = c->p.robin_top != 0, dmg = c->p.damage_every_step != 0;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -821,10 +817,8 @@ int tam_heat_run(tam_ctx* c, int32_t loops, tam_pulse* pulse)
             else c->time += c->p.delt;
             const double lon1 = (pulse && pulse->use_pulse) ? pulse->laserOn : 1.0;
             const double t_after1 = (pulse ? pulse->time : c->time) + c->p.delt;
-            a.sid = c->sid;
             HIPCHK(c, launch_heat_step2(a, c->cur, c->nxt, (float)lon, (float)lon1, t_after, t_after1, robin, dmg,
                                         1, c->tile, c->kz, c->stream));
-            c->sid += 2;
             std::swap(c->cur, c->nxt);
             if (pulse) tam_pulse_advance(pulse, c->p.delt);
             else c->time += c->p.delt;
@@ -835,12 +829,10 @@ int tam_heat_run(tam_ctx* c, int32_t loops, tam_pulse* pulse)
             HIPCHK(c, launch_heat64_step(a64, c->cur64, c->nxt64, lon, t_after, boil, robin, dmg, 1, c->stream));
             std::swap(c->cur64, c->nxt64);
         } else {
-            a.sid = c->sid;
             HIPCHK(c, launch_heat_step(a, c->cur, c->nxt, (float)lon, t_after, boil, robin, dmg, 1, c->lut ? 1 : 0,
                                        c->tile, c->kz, c->stream));
             std::swap(c->cur, c->nxt);
         }
-        c->sid += 1;
         if (pulse) tam_pulse_advance(pulse, c->p.delt);
         else c->time += c->p.delt;
     }
@@ -942,8 +934,6 @@ int tam_cycle(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream, doub
     }
     h->arr[0] = arr_delt;
     h->arr[1] = pulse ? pulse->time : c->time;
-    h->sid0 = c->sid;
-    c->sid += (uint32_t)loops;
 
     // ---- graph for the current ping-pong parity (captured on first use)
     const int par = c->fp64 ? (c->cur64 == c->T64a ? 0 : 1) : (c->cur == c->dT0 ? 0 : 1);
@@ -1101,11 +1091,12 @@ int tam_upload(tam_ctx* c, int32_t field, const double* in)
     }
     HIPCHK(c, hipMemcpyAsync(dst, in, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream));
     if (field == TAM_FIELD_THRESTIME || field == TAM_FIELD_TISSUE) {
-        // the mask's set bits and due chains follow the new times / Omega;
-        // every plane is checked in the next step
+        // the mask's set bits and due chains follow the new times / Omega,
+        // and the class bytes mirror the due chains
         HIPCHK(c, launch_tmask(c->thres, c->omega, c->p.arr_thr[0], c->p.arr_thr[1], c->p.arr_thr[2], c->tmask, V,
                                c->stream));
-        HIPCHK(c, launch_fill_u32(c->pflag, 3 * ((long)c->p.nz + 2), c->sid, c->stream));
+        if (c->lut)
+            HIPCHK(c, launch_class_due(c->cls, c->tmask, c->p.nx, c->p.ny, c->p.nz, c->px, c->pxy, c->stream));
     }
     if (field == TAM_FIELD_JMEAN || field == TAM_FIELD_COEFF) {
         if (refresh_src(c) != TAM_OK) return TAM_ERR_HIP;
@@ -1291,7 +1282,7 @@ int tam_tune_heat(tam_ctx* c, int32_t reps, int32_t* tile, int32_t* kz, double* 
         if (e == hipSuccess && !c->tn_eblk) e = dalloc(&c->tn_eblk, c->neblk);
         if (e == hipSuccess && !c->tn_Q) e = dalloc(&c->tn_Q, V);
         if (e == hipSuccess && !c->tn_tmask) e = dalloc(&c->tn_tmask, V);
-        if (e == hipSuccess && !c->tn_pflag) e = dalloc(&c->tn_pflag, 3 * ((long)c->p.nz + 2));
+        if (e == hipSuccess && c->lut && !c->tn_cls) e = dalloc(&c->tn_cls, c->nhalo);
         double *om = c->tn_omega, *th = c->tn_thres, *eb = c->tn_eblk, *qs = c->tn_Q;
         uint8_t* tm = c->tn_tmask;
         if (e == hipSuccess) e = hipMemcpyAsync(om, c->omega, sizeof(double) * (size_t)V, hipMemcpyDeviceToDevice,
@@ -1301,13 +1292,14 @@ int tam_tune_heat(tam_ctx* c, int32_t reps, int32_t* tile, int32_t* kz, double* 
         if (e == hipSuccess) e = hipMemcpyAsync(th, c->thres, sizeof(double) * 3 * (size_t)V,
                                                 hipMemcpyDeviceToDevice, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(tm, c->tmask, (size_t)V, hipMemcpyDeviceToDevice, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(c->tn_pflag, c->pflag, sizeof(uint32_t) * 3 * ((size_t)c->p.nz + 2),
-                                                hipMemcpyDeviceToDevice, c->stream);
+        // the steps keep the class bytes' due bits: candidates write a copy
+        if (e == hipSuccess && c->lut)
+            e = hipMemcpyAsync(c->tn_cls, c->cls, (size_t)c->nhalo, hipMemcpyDeviceToDevice, c->stream);
         if (e == hipSuccess) e = launch_fill_f64(eb, c->neblk, 0.0, c->stream);
         a.omega = om;
         a.thres = th;
         a.tmask = tm;
-        a.pflag = c->tn_pflag;
+        if (c->lut) a.cls = c->tn_cls;
         a.eblk = eb;
         a.Q = qs;
         const int boil = c->p.boil != 0, robin = c->p.robin_top != 0, dmg = c->p.damage_every_step != 0;
@@ -1490,7 +1482,6 @@ int batch_heat(tam_batch* b, int steps, int tile, int kz, bool advance)
         for (tam_ctx* c : b->ctx) {
             if (par & 1) std::swap(c->cur, c->nxt);
             c->time += steps * c->p.delt;
-            c->sid += (uint32_t)steps;
             c->timing.heat_steps = steps;
         }
     }

@@ -85,14 +85,17 @@ __device__ __forceinline__ TileId xcd_tile()
 }
 
 // One step's damage at interior voxel v, dT = tn after the step
-// (3dFD.f90:450-461): Omega grows inside the Arrhenius window and the
-// threshold chain runs there; outside it the chain runs where a threshold is
-// still due from the previous step (chk: the plane has such a voxel).  Omega
-// and the mask come from the prefetch when pf.  Returns true when a
-// threshold is still due after this step.
-__device__ __forceinline__ bool damage_one(const HeatArgs& a, long v, long nv, float tn, double t_after, bool chk,
-                                           bool pf, double om_pf, unsigned tm_pf)
+// (3dFD.f90:450-461): inside the Arrhenius window Omega grows and the
+// threshold chain runs; outside it the chain runs only where a time is due
+// (due_in: TAM_CLASS_DUE of the class byte, or the mask's TAM_TMASK_DUE
+// without classes).  Omega and the mask come from the prefetch when pf.
+// Returns whether a time is still due after this step.
+__device__ __forceinline__ bool damage_one(const HeatArgs& a, long v, long nv, float tn, double t_after, bool pf,
+                                           double om_pf, unsigned tm_pf, bool due_in)
 {
+    bool chain = due_in;                   // outside the window: only where a time is due
+    double om = 0.0;
+    bool have = false;
     if (tn >= a.dT_lo_test && tn < a.dT_hi_test) {
         const double T = a.t_ref + (double)tn;
         if (T >= a.arr_Tlo && T < a.arr_Thi) {
@@ -100,32 +103,19 @@ __device__ __forceinline__ bool damage_one(const HeatArgs& a, long v, long nv, f
             // FP64 (both terms ~230), exponentiated in FP32 (|arg| < ~40,
             // <= 2e-6 relative), accumulated in FP64
             const float arg = (float)(a.lnA - a.arr_dE / (a.arr_R * T));
-            const double om = (pf ? om_pf : a.omega[v]) + a.delt * (double)expf(arg);
+            om = (pf ? om_pf : a.omega[v]) + a.delt * (double)expf(arg);
             a.omega[v] = om;
+            have = true;
             // the threshold chain can only fire once Omega reaches the lowest
             // threshold (and nothing is due below it)
-            if (om < a.thr_min) return false;
-            const unsigned m = thres_chain(a.thres, a.tmask, v, nv, om, a.thr0, a.thr1, a.thr2, t_after,
-                                           pf ? tm_pf : a.tmask[v]);
-            return (m & TAM_TMASK_DUE) != 0u;
+            chain = om >= a.thr_min;
         }
     }
-    if (chk) {
-        const unsigned m0 = pf ? tm_pf : a.tmask[v];
-        if (m0 & TAM_TMASK_DUE) {
-            const unsigned m = thres_chain(a.thres, a.tmask, v, nv, pf ? om_pf : a.omega[v], a.thr0, a.thr1, a.thr2,
-                                           t_after, m0);
-            return (m & TAM_TMASK_DUE) != 0u;
-        }
-    }
-    return false;
-}
-
-// Step id of a launch: by value, or the graph replay's first step id plus
-// the launch's offset in the cycle.
-__device__ __forceinline__ uint32_t step_id(const HeatArgs& a)
-{
-    return a.dyn_sid ? *a.dyn_sid + a.sid : a.sid;
+    if (!chain) return false;
+    if (!have) om = pf ? om_pf : a.omega[v];
+    const unsigned m = thres_chain(a.thres, a.tmask, v, nv, om, a.thr0, a.thr1, a.thr2, t_after,
+                                   pf ? tm_pf : a.tmask[v]);
+    return (m & TAM_TMASK_DUE) != 0u;
 }
 
 template <int TX, int TY, bool LUT>
@@ -177,9 +167,14 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
     // z registers: dT and class (or gz) of the planes below / at / above
     float tm = Tin[c - pxy], tc = Tin[c], tp = Tin[c + pxy];
     uint8_t cm = 0, cc = 0, cp = 0;
+    unsigned ccr = 0, cpr = 0;                  // raw class bytes (TAM_CLASS_DUE) of planes k, k+1
     float gzm = 0.f, gzc = 0.f;
     if (LUT) {
-        cm = cla[c - pxy] & TAM_CLASS_MASK; cc = cla[c] & TAM_CLASS_MASK; cp = cla[c + pxy] & TAM_CLASS_MASK;
+        cm = cla[c - pxy] & TAM_CLASS_MASK;
+        ccr = cla[c];
+        cpr = cla[c + pxy];
+        cc = ccr & TAM_CLASS_MASK;
+        cp = cpr & TAM_CLASS_MASK;
     } else {
         gzm = gza[c - pxy]; gzc = gza[c];
     }
@@ -210,8 +205,6 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
     const long nxy = (long)nx * ny;
     long v = (long)(ic - 1) + (long)nx * (jc - 1) + nxy * (k0 - 1);
     const float wlo = a.dT_lo_test - a.dT_win_pad, whi = a.dT_hi_test + a.dT_win_pad;
-    const uint32_t sid = step_id(a);
-    const uint64_t dmask = damage ? due_planes(a.pflag, nz, sid, k0, k1) : 0ull;
     bool pom = damage && in && tc >= wlo && tc < whi;
     double om_pf = 0.0;
     unsigned tm_pf = 0;
@@ -241,6 +234,7 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
         }
         const float tc_k = tc, tm_k = tm, tp_k0 = tp, s_k = s;
         const int cc_k = cc, cm_k = cm, cp_k = cp;
+        const unsigned ccr_k = ccr;
         const float gzm_k = gzm, gzc_k = gzc;
         const bool pom_k = pom;
         const double om_k = om_pf;
@@ -255,7 +249,9 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
             if (hym) hT2 = Tin[n - px];
             if (hyp) hT3 = Tin[n + px];
             if (LUT) {
-                cm = cc; cc = cp; cp = cla[n + pxy] & TAM_CLASS_MASK;
+                cm = cc; cc = cp; ccr = cpr;
+                cpr = cla[n + pxy];
+                cp = cpr & TAM_CLASS_MASK;
                 if (hxm) hC0 = cla[n - 1] & TAM_CLASS_MASK;
                 if (hxp) hC1 = cla[n + 1] & TAM_CLASS_MASK;
                 if (hym) hC2 = cla[n - px] & TAM_CLASS_MASK;
@@ -274,7 +270,6 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
             }
         }
         __syncthreads();
-        const bool chk = (dmask >> (k - k0)) & 1ull;
         if (!in) continue;
         // ---- update of voxel (i, j, k)
         const float txm = sT[ty + 1][tx], txp = sT[ty + 1][tx + 2];
@@ -318,9 +313,11 @@ __global__ __launch_bounds__(TX* TY) void stencil_kernel(const HeatArgs a, const
             }
         }
         Tout[c] = tn;
-        if (damage) {                                                // 3dFD.f90:450-461, every step
-            const bool due = damage_one(a, v, nxy * nz, tn, t_after, chk, pom_k, om_k, tmask_k);
-            mark_plane_due(a.pflag, nz, sid + 1u, k, due);
+        if (damage) {                                               // 3dFD.f90:450-461, every step
+            const bool din = LUT ? (ccr_k & TAM_CLASS_DUE) != 0u
+                                 : ((pom_k ? tmask_k : (unsigned)a.tmask[v]) & TAM_TMASK_DUE) != 0u;
+            const bool dout = damage_one(a, v, nxy * nz, tn, t_after, pom_k, om_k, tmask_k, din);
+            if (LUT && dout != din) a.cls[c] = (uint8_t)((ccr_k & ~TAM_CLASS_DUE) | (dout ? TAM_CLASS_DUE : 0u));
         }
         if (energy && (bnd_ij || k == 1 || k == nz)) {
             float e = 0.f;
@@ -397,8 +394,6 @@ __device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __re
         gzm = gza[c - pxy];
     }
     float e_acc = 0.f;
-    const uint32_t sid = step_id(a);
-    const uint64_t dmask = damage ? due_planes(a.pflag, nz, sid, k0, k1) : 0ull;
     for (int k = k0; k < k1; ++k, c += pxy, v += nxy) {
         const float tp0 = Tin[c + pxy];
         const float txm = Tin[c - 1], txp = Tin[c + 1], tym = Tin[c - px], typ = Tin[c + px];
@@ -445,7 +440,6 @@ __device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __re
         }
         const float inc = face_sum(tc, gxm, txm, gxp, txp, gym, tym, gyp, typ, gzm, tm, gzp, tp);
         float tn = step_update(tc, inc, lon, s_k);
-        bool due = false;
         if (in) {
             if (tc < a.dT_neg) a.flags[0] = 1;                   // 3dFD.f90:179-182
             if (boil && tc >= a.dT_boil_test) {                   // 3dFD.f90:167-172
@@ -461,8 +455,12 @@ __device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __re
                 }
             }
             Tout[c] = tn;
-            if (damage)                                                  // 3dFD.f90:450-461
-                due = damage_one(a, v, nxy * nz, tn, t_after, (dmask >> (k - k0)) & 1ull, pom, om_pf, tm_pf);
+            if (damage) {                                                // 3dFD.f90:450-461
+                const bool din = LUT ? (ccr & TAM_CLASS_DUE) != 0u
+                                     : ((pom ? tm_pf : (unsigned)a.tmask[v]) & TAM_TMASK_DUE) != 0u;
+                const bool dout = damage_one(a, v, nxy * nz, tn, t_after, pom, om_pf, tm_pf, din);
+                if (LUT && dout != din) a.cls[c] = (uint8_t)((ccr & ~TAM_CLASS_DUE) | (dout ? TAM_CLASS_DUE : 0u));
+            }
             if (energy && (bnd_ij || k == 1 || k == nz)) {
                 float e = 0.f;
                 if (i == 1) e += gxm * (txm - tc);
@@ -474,7 +472,6 @@ __device__ __forceinline__ void direct_body(const HeatArgs& a, const float* __re
                 e_acc -= (LUT ? sLut[768 + cc] : (float)(a.heatcap[c] * a.mass_voxel)) * e;
             }
         }
-        if (damage) mark_plane_due(a.pflag, nz, sid + 1u, k, due);
         tm = tc;
         tc = tp0;
         if (LUT) {
@@ -514,13 +511,15 @@ __global__ __launch_bounds__(TX* TY) void stencil_direct_kernel(const HeatArgs a
 // (3dFD.f90:450-461, the same arithmetic as the one-step kernels).  Omega and
 // the threshold mask are loaded on first use (from the prefetch when pom)
 // and kept in registers across the two steps of a launch; `dirty` says that
-// Omega changed and must be stored.  Outside the window the chain runs where
-// a threshold is due: from the previous launch when chk (the plane has such a
-// voxel), or from the launch's first step (the mask is then in registers).
-__device__ __forceinline__ void damage_step(const HeatArgs& a, long v, long nv, float tn, double t_after, bool chk,
+// Omega changed and must be stored.  Outside the window the chain runs only
+// where a time is due (`due`: from the class byte for the launch's first
+// step, from the first step's chain for the second).  Returns whether a time
+// is still due after the step.
+__device__ __forceinline__ bool damage_step(const HeatArgs& a, long v, long nv, float tn, double t_after, bool due,
                                             bool pom, double om_pf, unsigned tm_pf, double& om, bool& have,
                                             bool& dirty, unsigned& m, bool& have_m)
 {
+    bool chain = due;                      // outside the window: only where a time is due
     if (tn >= a.dT_lo_test && tn < a.dT_hi_test) {
         const double T = a.t_ref + (double)tn;
         if (T >= a.arr_Tlo && T < a.arr_Thi) {
@@ -531,29 +530,20 @@ __device__ __forceinline__ void damage_step(const HeatArgs& a, long v, long nv, 
             }
             om = om + a.delt * (double)expf(arg);
             dirty = true;
-            if (om >= a.thr_min) {
-                if (!have_m) {
-                    m = pom ? tm_pf : a.tmask[v];
-                    have_m = true;
-                }
-                m = thres_chain(a.thres, a.tmask, v, nv, om, a.thr0, a.thr1, a.thr2, t_after, m);
-            }
-            return;
+            chain = om >= a.thr_min;
         }
     }
-    if (chk || have_m) {
-        if (!have_m) {
-            m = pom ? tm_pf : a.tmask[v];
-            have_m = true;
-        }
-        if (m & TAM_TMASK_DUE) {
-            if (!have) {
-                om = pom ? om_pf : a.omega[v];
-                have = true;
-            }
-            m = thres_chain(a.thres, a.tmask, v, nv, om, a.thr0, a.thr1, a.thr2, t_after, m);
-        }
+    if (!chain) return false;
+    if (!have) {
+        om = pom ? om_pf : a.omega[v];
+        have = true;
     }
+    if (!have_m) {
+        m = pom ? tm_pf : a.tmask[v];
+        have_m = true;
+    }
+    m = thres_chain(a.thres, a.tmask, v, nv, om, a.thr0, a.thr1, a.thr2, t_after, m);
+    return (m & TAM_TMASK_DUE) != 0u;
 }
 
 // Two steps per launch (temporal blocking), class mode without boiling.
@@ -608,8 +598,6 @@ __device__ __forceinline__ void t2_body(const HeatArgs& a, const float* __restri
     __syncthreads();
 
     const long nxy = (long)nx * ny, nv = nxy * nz;
-    const uint32_t sid = step_id(a);                             // id of the launch's first step
-    const uint64_t dmask = damage ? due_planes(a.pflag, nz, sid, k0, k1) : 0ull;
     const long vcol = (long)(i - 1) + (long)nx * (j - 1);        // interior index of the column (own lanes)
     const float wlo = a.dT_lo_test - a.dT_win_pad, whi = a.dT_hi_test + a.dT_win_pad;
     // first step at plane p: dT0 at p-1, p (z0m, z0c), classes at p-1, p
@@ -623,6 +611,7 @@ __device__ __forceinline__ void t2_body(const HeatArgs& a, const float* __restri
     float u1m = 0.f, u1c = 0.f;
     float Gxm = 0.f, Gxp = 0.f, Gym = 0.f, Gyp = 0.f, Gzm = 0.f, Gzp = 0.f, Sk = 0.f;
     int Ck = 0;
+    unsigned Ckr = 0;                                         // raw class byte at k (TAM_CLASS_DUE)
     float e_acc = 0.f;
     // loads of plane q (x-y neighbours, source) and of the column at q+1,
     // issued one iteration ahead of their use
@@ -702,8 +691,6 @@ __device__ __forceinline__ void t2_body(const HeatArgs& a, const float* __restri
             }
         }
         // ---- second step at plane k = p-1 (dT1 of plane k is in sT1[k & 1])
-        const bool chk = step2 && ((dmask >> (k - k0)) & 1ull);
-        bool due2 = false;
         if (step2 && own) {
             const float xm = sT1[k & 1][ty][tx - 1], xp = sT1[k & 1][ty][tx + 1];
             const float ym = sT1[k & 1][ty - 1][tx], yp = sT1[k & 1][ty + 1][tx];
@@ -720,10 +707,12 @@ __device__ __forceinline__ void t2_body(const HeatArgs& a, const float* __restri
                 double om = 0.0;
                 unsigned m = 0;
                 bool have = false, dirty = false, have_m = false;
-                damage_step(a, v, nv, u1c, ta0, chk, pom, om_pf, tm_pf, om, have, dirty, m, have_m);
-                damage_step(a, v, nv, tn, ta1, false, pom, om_pf, tm_pf, om, have, dirty, m, have_m);
+                const bool din = (Ckr & TAM_CLASS_DUE) != 0u;
+                bool due = damage_step(a, v, nv, u1c, ta0, din, pom, om_pf, tm_pf, om, have, dirty, m, have_m);
+                due = damage_step(a, v, nv, tn, ta1, due, pom, om_pf, tm_pf, om, have, dirty, m, have_m);
                 if (dirty) a.omega[v] = om;
-                due2 = have_m && (m & TAM_TMASK_DUE);
+                if (due != din)
+                    a.cls[ch + pxy * k] = (uint8_t)((Ckr & ~TAM_CLASS_DUE) | (due ? TAM_CLASS_DUE : 0u));
             }
             if (energy && (bnd_ij || k == 1 || k == nz)) {
                 float e = 0.f;
@@ -736,7 +725,6 @@ __device__ __forceinline__ void t2_body(const HeatArgs& a, const float* __restri
                 e_acc -= sLut[768 + Ck] * e;
             }
         }
-        if (step2 && damage) mark_plane_due(a.pflag, nz, sid + 2u, k, due2);
         // plane p's buffer last held plane p-2, whose readers passed the
         // previous iteration's barrier
         sT1[p & 1][ty][tx] = t1p;
@@ -746,6 +734,7 @@ __device__ __forceinline__ void t2_body(const HeatArgs& a, const float* __restri
         z0m = z0c;
         z0c = z0p;
         cm = cc;
+        Ckr = ccr;                                            // plane p's byte, before the rotation
         ccr = cpr;
         Gxm = gxm; Gxp = gxp; Gym = gym; Gyp = gyp; Gzm = gzm; Gzp = gzp;
         Sk = s_p;
@@ -973,7 +962,6 @@ hipError_t launch_heat_step2(const HeatArgs& a, const float* Tin, float* Tout, f
 {
     const unsigned f = (robin ? F_ROBIN : 0u) | (damage ? F_DAMAGE : 0u) | (energy ? F_ENERGY : 0u);
     if (kz < 1) kz = 1;
-    if (kz > TAM_MAX_KZ) kz = TAM_MAX_KZ;
     if (kz > a.nz) kz = a.nz;
     if (!a.cls || !a.glut) return hipErrorInvalidValue;      // class mode only
     switch (tile) {
@@ -991,7 +979,6 @@ hipError_t launch_heat_step(const HeatArgs& a, const float* Tin, float* Tout, fl
     const unsigned f = (boil ? F_BOIL : 0u) | (robin ? F_ROBIN : 0u) | (damage ? F_DAMAGE : 0u) |
                        (energy ? F_ENERGY : 0u);
     if (kz < 1) kz = 1;
-    if (kz > TAM_MAX_KZ) kz = TAM_MAX_KZ;
     if (kz > a.nz) kz = a.nz;
     if (tile == 6 || tile == 7) tile = 4;       // two-step shapes: one step on direct-32x8 / direct-64x8
     else if (tile == 8) tile = 5;

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void props_kernel(RebuildArgs a)   // 3dFD.f90
         a.coeff[c] = a.delt / (air_density(T) * hc);
         a.mut[v] = 0.f;
         a.mat[v] = (uint8_t)a.air_id;
-        if (a.cls) a.cls[c] = (uint8_t)TAM_CLASS_REMOVED;
+        if (a.cls) a.cls[c] = (uint8_t)(TAM_CLASS_REMOVED | (a.cls[c] & TAM_CLASS_DUE));
     } else if (a.mode == TAM_THERMAL_WATER) {
         const double W = a.W[v];
         const double rho = skin_density(W);
@@ -301,10 +301,7 @@ __global__ __launch_bounds__(256) void tmask_kernel(const double* __restrict__ t
     const bool due = (!(m & 1u) && om >= thr0) || (!(m & 2u) && om >= thr1) || (!(m & 4u) && om >= thr2);
     tmask[v] = (uint8_t)(m | (due ? TAM_TMASK_DUE : 0u));
 }
-__global__ void fill_u32_kernel(uint32_t* p, long n, uint32_t v)
-{
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
-}
+
 }  // namespace
 
 namespace {
@@ -317,14 +314,36 @@ __global__ __launch_bounds__(256) void class_flags_kernel(uint8_t* cls, int nx, 
     const int j = (int)(r % ny) + 1, k = (int)(r / ny) + 1;
     const long c = i + px * j + pxy * k;
     // only bit 7 of a byte changes here, so the neighbours' classes read the
-    // same whether or not their own flag was rewritten first
-    const unsigned m = cls[c] & TAM_CLASS_MASK;
+    // same whether or not their own flag was rewritten first; the due bit stays
+    const unsigned m = cls[c] & TAM_CLASS_MASK, due = cls[c] & TAM_CLASS_DUE;
     const bool hom = (cls[c - 1] & TAM_CLASS_MASK) == m && (cls[c + 1] & TAM_CLASS_MASK) == m &&
                      (cls[c - px] & TAM_CLASS_MASK) == m && (cls[c + px] & TAM_CLASS_MASK) == m &&
                      (cls[c - pxy] & TAM_CLASS_MASK) == m && (cls[c + pxy] & TAM_CLASS_MASK) == m;
-    cls[c] = (uint8_t)(m | (hom ? TAM_CLASS_HOM : 0u));
+    cls[c] = (uint8_t)(m | due | (hom ? TAM_CLASS_HOM : 0u));
+}
+
+__global__ __launch_bounds__(256) void class_due_kernel(uint8_t* cls, const uint8_t* tmask, int nx, int ny, int nz,
+                                                        long px, long pxy)
+{
+    const long v = (long)blockIdx.x * 256 + threadIdx.x;
+    if (v >= (long)nx * ny * nz) return;
+    const int i = (int)(v % nx) + 1;
+    const long r = v / nx;
+    const int j = (int)(r % ny) + 1, k = (int)(r / ny) + 1;
+    const long c = i + px * j + pxy * k;
+    cls[c] = (uint8_t)((cls[c] & ~TAM_CLASS_DUE) | ((tmask[v] & TAM_TMASK_DUE) ? TAM_CLASS_DUE : 0u));
 }
 }  // namespace
+
+hipError_t launch_class_due(uint8_t* cls, const uint8_t* tmask, int nx, int ny, int nz, long px, long pxy,
+                            hipStream_t s)
+{
+    const long nv = (long)nx * ny * nz;
+    if (nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(class_due_kernel, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, cls, tmask, nx, ny, nz,
+                       px, pxy);
+    return hipGetLastError();
+}
 
 hipError_t launch_class_flags(uint8_t* cls, int nx, int ny, int nz, long px, long pxy, hipStream_t s)
 {
@@ -344,12 +363,6 @@ hipError_t launch_tmask(const double* thres, const double* omega, double thr0, d
     return hipGetLastError();
 }
 
-hipError_t launch_fill_u32(uint32_t* p, long n, uint32_t v, hipStream_t s)
-{
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_fill(n)), dim3(256), 0, s, p, n, v);
-    return hipGetLastError();
-}
 
 hipError_t launch_fill_f64(double* p, long n, double v, hipStream_t s)
 {

@@ -68,37 +68,17 @@ __device__ __forceinline__ unsigned thres_chain(double* thres, uint8_t* tmask, l
     if (!(m & 1u) && om >= thr0) k = 0;
     else if (!(m & 2u) && om >= thr1) k = 1;
     else if (!(m & 4u) && om >= thr2) k = 2;
+    m &= ~TAM_TMASK_DUE;                   // nothing written: nothing is due either
     if (k >= 0) {
         thres[v + k * nv] = t;
         if (t != 0.0) m |= 1u << k;
+        if ((!(m & 1u) && om >= thr0) || (!(m & 2u) && om >= thr1) || (!(m & 4u) && om >= thr2))
+            m |= TAM_TMASK_DUE;
     }
-    const bool due = (!(m & 1u) && om >= thr0) || (!(m & 2u) && om >= thr1) || (!(m & 4u) && om >= thr2);
-    m = due ? (m | TAM_TMASK_DUE) : (m & ~TAM_TMASK_DUE);
     if (m != m0) tmask[v] = (uint8_t)m;
     return m;
 }
 
-// Planes with a due chain: the step of id s reads slot s % 3 of the
-// 3 x (nz + 2) step-tagged plane flags and checks plane k when the flag holds
-// s; a launch whose last step is s' tags the planes where it left a chain due
-// with s' + 1 in slot (s' + 1) % 3.  One- and two-step launches thus never
-// read the slot they write, and a stale tag never matches a later step.
-// A block reads the flags of its planes k0 .. k1-1 (at most 64) once, one
-// plane per lane, into a wave-uniform mask: bit k - k0 says plane k is due.
-__device__ __forceinline__ uint64_t due_planes(const uint32_t* pflag, int nz, uint32_t sid, int k0, int k1)
-{
-    const int t = (int)(threadIdx.x & 63u);
-    const bool f = t < k1 - k0 && pflag[(size_t)(sid % 3u) * (size_t)(nz + 2) + (size_t)(k0 + t)] == sid;
-    return __ballot(f);
-}
-__device__ __forceinline__ void mark_plane_due(uint32_t* pflag, int nz, uint32_t sid_next, int k, bool due)
-{
-    const uint64_t w = __ballot(due);
-    if (w && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)w) - 1))
-        pflag[(size_t)(sid_next % 3u) * (size_t)(nz + 2) + (size_t)k] = sid_next;
-}
-// the launch shapes' largest plane count per block (due_planes reads one plane per lane)
-constexpr int TAM_MAX_KZ = 64;
 
 struct MatEntry {          // per material, in device memory
     float albedo, g, n, mua;
@@ -166,8 +146,6 @@ struct HeatArgs {
     double* omega;             // interior
     double* thres;             // interior x3
     uint8_t* tmask;            // interior: bit k set when thres[k] != 0, TAM_TMASK_DUE
-    uint32_t* pflag;           // 3 x (nz+2) step-tagged planes with a due chain (see plane_due)
-    uint32_t sid;              // id of this launch's (first) step
     const double* rhokap;      // halo
     double arr_A, arr_dE, arr_R, arr_Tlo, arr_Thi, thr0, thr1, thr2;
     double lnA;                // log(arr_A), for the in-step damage rate
@@ -178,7 +156,7 @@ struct HeatArgs {
     double* eblk;              // per-block boundary outflow slots (FP32 stencil)
     int* flags;                // [0] negative temperature seen
     // class mode (see heat.hip)
-    const uint8_t* cls;        // halo: thermal class per cell
+    uint8_t* cls;              // halo: thermal class per cell (the step keeps TAM_CLASS_DUE)
     const float* glut;         // [3][16][16] face coefficient per (axis, minus class, plus class)
     const float2* robin_lut;   // [16] convective ghost (beta, gamma') per top-cell class
     // FP32 pre-tests (conservative) before the exact FP64 tests
@@ -188,7 +166,6 @@ struct HeatArgs {
     const float* dyn_lon;
     const double* dyn_tafter;
     const double* dyn_arr;
-    const uint32_t* dyn_sid;   // graph replays: the cycle's first step id; step id = *dyn_sid + sid
 };
 
 // one instance of a batched heat step: its arguments and its ping-pong pair
@@ -205,6 +182,10 @@ struct HeatBatchEntry {
 // take the class as (byte & TAM_CLASS_MASK).
 #define TAM_CLASS_HOM 0x80u
 #define TAM_CLASS_MASK 0x0fu
+// bit 6 of an interior class byte mirrors its mask's TAM_TMASK_DUE: the step
+// kernels read the class byte of every voxel anyway, so the rare voxel with a
+// threshold time due outside the Arrhenius window costs the others nothing
+#define TAM_CLASS_DUE 0x40u
 
 struct Heat64Args {            // FP64 reference-order step (heat64.hip)
     int nx, ny, nz;
@@ -298,12 +279,10 @@ struct tam_ctx {
     double* mua_frac;           // per material mu_a / mu_t (FP64)
     double *tally, *jmean, *Q, *W, *omega, *thres;
     uint8_t* tmask;             // bit k: thres[k] != 0 (kept with every write of thres), TAM_TMASK_DUE
-    uint32_t* pflag;            // 3 x (nz+2) step-tagged planes with a due threshold chain
-    uint32_t sid;               // id of the next heat step (steps since creation)
     // heat tuner scratch copies (allocated on the first tam_tune_heat, kept)
     double *tn_omega, *tn_thres, *tn_Q, *tn_eblk;
     uint8_t* tn_tmask;
-    uint32_t* tn_pflag;
+    uint8_t* tn_cls;
     double* box_psum;           // LDS tally box placement: per-plane sums (nz) and the chosen
     int* box_geom;              // geometry (o[3], n[3])
     double* energy;             // [0] boundary outflow, [1..] reduction scratch
@@ -336,9 +315,7 @@ struct CycleHdr {
     int32_t mc_on;             // 1: run transport + normalise
     double scale;              // normalisation scale
     double arr[2];             // Arrhenius delt, time (reference cadence)
-    uint32_t sid0;             // id of the cycle's first heat step
-    uint32_t pad0;
-    double pad[2];
+    double pad[3];
 };
 static_assert(sizeof(CycleHdr) == 64, "cycle header layout");
 
@@ -401,7 +378,9 @@ int tam_fail_noctx(int code, const std::string& msg);
 // TAM_TMASK_DUE where a threshold with an unset time is already reached
 hipError_t launch_tmask(const double* thres, const double* omega, double thr0, double thr1, double thr2,
                         uint8_t* tmask, long nvox, hipStream_t s);
-hipError_t launch_fill_u32(uint32_t* p, long n, uint32_t v, hipStream_t s);
+// TAM_CLASS_DUE bits of the interior class bytes from the masks' TAM_TMASK_DUE
+hipError_t launch_class_due(uint8_t* cls, const uint8_t* tmask, int nx, int ny, int nz, long px, long pxy,
+                            hipStream_t s);
 hipError_t launch_fill_f32(float* p, long n, float v, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, long n, hipStream_t s);
 hipError_t launch_copy(const void* in, void* out, long bytes, hipStream_t s);
