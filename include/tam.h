@@ -255,6 +255,15 @@ int tam_cycle(tam_ctx* ctx, int64_t npackets, uint64_t seed, uint32_t stream, do
  * edge), x-block fastest: a cheap device-side summary for checks that
  * compare two transport runs (no reference counterpart). */
 int tam_tally_blocks(tam_ctx* ctx, int32_t k, double* sums);
+/* tam_tally_blocks with no device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  The sums go through a fixed
+ * device buffer of TAM_BLOCKSUM_SCRATCH_MAX doubles (2 MiB) that each GPU
+ * the library runs on holds until the process ends; more blocks than that
+ * is TAM_ERR_ARG.  Every call on one GPU shares that GPU's buffer: calls
+ * that overlap on one GPU (from several threads or contexts) race. */
+#define TAM_BLOCKSUM_SCRATCH_MAX 262144
+int tam_tally_blocks_noalloc(tam_ctx* ctx, int32_t k, double* sums);
 int tam_download(tam_ctx* ctx, int32_t field, double* out);
 int tam_upload(tam_ctx* ctx, int32_t field, const double* in);
 

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -31,6 +32,37 @@ int fail(tam_ctx* c, int code, const std::string& msg)
 
 int tam_fail_noctx(int code, const std::string& msg) { return fail(nullptr, code, msg); }
 
+// ---- fixed device buffer of tam_tally_blocks_noalloc -----------------------
+// Every GPU the library runs on holds its own copy of `scratch` until the
+// process ends.  In host code `scratch` names a host-side stand-in, not a
+// device address: the device address comes from hipGetSymbolAddress, taken
+// once per device with that device current (at tam_create, or at the entry
+// point's first call on a device) and kept in scratch_dev[].
+static __device__ __align__(256) double scratch[TAM_BLOCKSUM_SCRATCH_MAX];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+double* scratch_dev[kMaxDevices] = {};
+
+// the calling thread's current device must be `dev`
+hipError_t scratch_ptr(int dev, double** out)
+{
+    *out = nullptr;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_dev[dev] = (double*)p;
+    }
+    *out = scratch_dev[dev];
+    return hipSuccess;
+}
+
+}  // namespace
 
 namespace {
 
@@ -577,6 +609,10 @@ int tam_create(const tam_params* p, tam_ctx** out)
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e == hipSuccess) e = hipGetLastError();
     if (e == hipSuccess) e = mc_prepare(c->dev);
+    if (e == hipSuccess) {
+        double* sp = nullptr;
+        e = scratch_ptr(c->dev, &sp);
+    }
     if (e != hipSuccess) {
         std::string m = std::string("device init: ") + hipGetErrorString(e);
         tam_destroy(c);
@@ -989,6 +1025,23 @@ int tam_tally_blocks(tam_ctx* c, int32_t k, double* sums)
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(tmp);
     if (e != hipSuccess) return hip_fail(c, "tally blocks", e);
+    return TAM_OK;
+}
+
+int tam_tally_blocks_noalloc(tam_ctx* c, int32_t k, double* sums)
+{
+    if (!c || !sums) return TAM_ERR_ARG;
+    if (k < 1 || c->p.nx % k || c->p.ny % k || c->p.nz % k) return fail(c, TAM_ERR_ARG, "k must divide every grid edge");
+    const long nb = (long)(c->p.nx / k) * (c->p.ny / k) * (c->p.nz / k);
+    if (nb > TAM_BLOCKSUM_SCRATCH_MAX)
+        return fail(c, TAM_ERR_ARG, "more blocks than the fixed buffer holds (TAM_BLOCKSUM_SCRATCH_MAX)");
+    Dev d(c->dev);
+    double* sp = nullptr;
+    HIPCHK(c, scratch_ptr(c->dev, &sp));
+    // every block sum is written (none accumulated), so the buffer needs no clearing
+    HIPCHK(c, launch_block_sums(c->tally, c->p.nx, c->p.ny, c->p.nz, k, sp, c->stream));
+    HIPCHK(c, hipMemcpyAsync(sums, sp, sizeof(double) * (size_t)nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return TAM_OK;
 }
 

@@ -31,7 +31,7 @@ EXPORTED = ("tam_abi_version", "tam_create", "tam_destroy", "tam_last_error", "t
             "tam_copy_bandwidth", "tam_atomic_rate", "tam_struct_size", "tam_pulse_advance", "tam_cycle",
             "tam_tally_blocks", "tam_batch_create", "tam_batch_destroy", "tam_batch_last_error",
             "tam_batch_mc_run", "tam_batch_heat_run", "tam_batch_set_launch", "tam_batch_tune", "tam_batch_last_ms",
-            "tam_spectrum_cdf", "tam_spectrum_sample", "tam_spectrum_interp")
+            "tam_spectrum_cdf", "tam_spectrum_sample", "tam_spectrum_interp", "tam_tally_blocks_noalloc")
 
 
 class Params(C.Structure):
@@ -132,6 +132,7 @@ def load():
     L.tam_copy_bandwidth.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
     L.tam_atomic_rate.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(C.c_double)]
     L.tam_tally_blocks.argtypes = [vp, C.c_int32, dp]
+    L.tam_tally_blocks_noalloc.argtypes = [vp, C.c_int32, dp]
     L.tam_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
     L.tam_batch_destroy.argtypes = [vp]
     L.tam_batch_destroy.restype = None
