@@ -26,6 +26,8 @@ Auxiliary legs for profiling (each prints its own JSON line):
   --leg stencil-state save PATH   run 3 headline cycles and save the tissue state
   --leg stencil-state run PATH    reload it and run --heat-steps steps per shape
                                   (--shapes tile:kz,...), after a 256 MiB copy
+  --leg transport-state run PATH  transport runs on such a state (crater phase)
+                                  at a fixed launch (--mc-launch, --mc-runs)
   --leg denoiser                  time forward passes and training steps of the
                                   denoiser at --grid (a multiple of 32)
 """
@@ -239,8 +241,7 @@ def c4_leg(args, log):
     n = args.c4_grid
     prob = configs.config4_tissue(n=(n, n, n), packets=int(1e8), steps=args.c4_steps, cycles=args.c4_cycles)
     dn = Denoiser((n, n, n), capacity=0)
-    dn.set_params(np.fromfile(os.path.join(ROOT, "tissue-ablation-mc_amd", "weights", "denoiser_c3.f32"),
-                              dtype=np.float32))
+    dn.set_params(np.fromfile(args.dn_weights, dtype=np.float32))
     kw = dict(cycles=args.c4_cycles, steps=args.c4_steps, hi=int(1e8), lo=int(1e6), tol=0.02)
     plain = coupled.run_form("plain", prob, **kw)
     log(f"c4 plain: {plain['wall_s']:.2f} s")
@@ -325,6 +326,43 @@ def stencil_state_leg(args, log):
     return res
 
 
+def transport_state_leg(args, log):
+    """Transport on a saved coupled-cycle tissue state (the crater phase) at a
+    fixed launch shape, --mc-runs runs of --packets packets, so a counter run
+    of the transport kernel sees the state the timed cycles spend most of
+    their transport time in.  The first run places the LDS tally box at the
+    top face (no path yet); later runs place it from the run before."""
+    import numpy as np
+
+    from tam_amd import configs, lib
+
+    n = args.grid
+    prob = configs.config1(n=(n, n, n), packets=int(args.packets), steps=10000, cycles=3)
+    d = np.load(args.state[1])
+    g = lib.TissueGrid(prob.params)
+    g.set_optics(prob.mat, prob.table, d["rhokap"])              # removed voxels: rhokap 0
+    g.set_thermal(d["temp"], d["kappa"], d["density"], d["heatcap"])
+    bpc, batch = (int(x) for x in args.mc_launch.split(":"))
+    g.set_mc_launch(bpc, batch)
+    g.set_mc_box(True)
+    # a streaming copy of known size (256 MiB each way): the calibration of
+    # the byte counters in a counter run of this leg
+    copy_gbps = g.copy_bandwidth(256 << 20, reps=3)
+    runs = []
+    for r in range(args.mc_runs):
+        st = g.mc_run(int(args.packets), seed=2024, stream=r)
+        npk = float(args.packets)
+        runs.append({"kernel_ms": st.kernel_ms, "packets_per_s": npk / (st.kernel_ms * 1e-3),
+                     "device_adds_per_packet": (st.tally_adds - st.box_adds) / npk,
+                     "box_adds_per_packet": st.box_adds / npk, "crossings_per_packet": st.crossings / npk,
+                     "interactions_per_packet": st.interactions / npk, "box": g.mc_box()})
+        log(f"transport run {r}: {runs[-1]['kernel_ms']:.1f} ms, {runs[-1]['device_adds_per_packet']:.1f} device "
+            f"+ {runs[-1]['box_adds_per_packet']:.1f} box adds per packet")
+    g.close()
+    return {"leg": "transport-state", "grid": n, "packets": int(args.packets), "blocks_per_cu": bpc,
+            "rng_batch": batch, "copy_gbps": copy_gbps, "runs": runs}
+
+
 def denoiser_leg(args, log):
     """Forward passes and training steps of the denoiser on random feature
     maps (the committed weights), timed by the network's own device timer."""
@@ -336,8 +374,7 @@ def denoiser_leg(args, log):
     n = args.grid
     slots = args.dn_slots if args.dn_train_steps > 0 else 0
     dn = Denoiser((n, n, n), capacity=slots, seed=1)
-    dn.set_params(np.fromfile(os.path.join(ROOT, "tissue-ablation-mc_amd", "weights", "denoiser_c3.f32"),
-                              dtype=np.float32))
+    dn.set_params(np.fromfile(args.dn_weights, dtype=np.float32))
     rng = np.random.default_rng(0)
     for q in range(slots):
         f = rng.random((4, n, n, n)).astype(np.float32)
@@ -378,9 +415,13 @@ def main():
     ap.add_argument("--c4-grid", type=int, default=256)
     ap.add_argument("--c4-cycles", type=int, default=20)
     ap.add_argument("--c4-steps", type=int, default=5000)
-    ap.add_argument("--leg", default="main", choices=("main", "stencil-state", "denoiser"))
+    ap.add_argument("--leg", default="main", choices=("main", "stencil-state", "transport-state", "denoiser"))
     ap.add_argument("--state", nargs=2, metavar=("save|run", "PATH"), default=None)
     ap.add_argument("--shapes", default="4:8", help="stencil-state leg: comma list of tile:kz")
+    ap.add_argument("--mc-launch", default="4:32", help="transport-state leg: blocks per CU:random-phase batch")
+    ap.add_argument("--mc-runs", type=int, default=3, help="transport-state leg: transport runs")
+    ap.add_argument("--dn-weights", default=os.path.join(ROOT, "tissue-ablation-mc_amd", "weights",
+                                                         "denoiser_c3.f32"), help="denoiser parameter vector")
     ap.add_argument("--dn-forward", type=int, default=5)
     ap.add_argument("--dn-train-steps", type=int, default=200)
     ap.add_argument("--dn-slots", type=int, default=8)
@@ -397,6 +438,11 @@ def main():
         if args.state is None or args.state[0] not in ("save", "run"):
             ap.error("--leg stencil-state needs --state save|run PATH")
         print(json.dumps(stencil_state_leg(args, log)), flush=True)
+        return
+    if args.leg == "transport-state":
+        if args.state is None or args.state[0] != "run":
+            ap.error("--leg transport-state needs --state run PATH (a state saved by the stencil-state leg)")
+        print(json.dumps(transport_state_leg(args, log)), flush=True)
         return
     if args.leg == "denoiser":
         print(json.dumps(denoiser_leg(args, log)), flush=True)

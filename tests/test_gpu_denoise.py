@@ -1,10 +1,13 @@
 """The denoiser's BF16 MFMA kernels against the CPU reference (torch float32
 conv3d / autograd with the same BF16 rounding of stored activations).
 
-Tolerances: forward prediction within 2% of the prediction's range (BF16
-rounding of ten layers' activations can differ by one BF16 ulp where the
-FP32 sums differ in order); gradients per parameter tensor with cosine
-similarity >= 0.99 and relative L2 error <= 6% (BF16 activation gradients).
+Tolerances (about twice the differences measured on MI355X): forward
+prediction within 0.6% of the prediction's range at the worst voxel and
+5e-4 of it at the median (measured 0.30% and 1.8e-4: BF16 rounding of ten
+layers' activations can differ by one BF16 ulp where the FP32 sums differ in
+order); loss within 1e-4 relative (measured 2e-6); gradients per parameter
+tensor with cosine similarity >= 0.9999 and relative L2 error <= 6e-3
+(measured cos 1.00000, relative error <= 3.0e-3: BF16 activation gradients).
 """
 import numpy as np
 import pytest
@@ -43,8 +46,8 @@ def test_forward_matches_cpu_reference(shape):
     span = ref.max() - ref.min()
     err = np.abs(out - ref)
     _print(f"range {span:.3f}, max abs err {err.max():.4f}, median {np.median(err):.2e}")
-    assert err.max() <= 0.02 * span
-    assert np.median(err) <= 2e-3 * span
+    assert err.max() <= 6e-3 * span
+    assert np.median(err) <= 5e-4 * span
 
 
 def test_gradients_match_cpu_autograd():
@@ -58,7 +61,7 @@ def test_gradients_match_cpu_autograd():
     gr, loss_r = R.gradients(p, f, t)
     dn.close()
     _print(f"loss gpu {loss:.6f} cpu {loss_r:.6f}")
-    assert abs(loss - loss_r) <= 1e-2 * loss_r
+    assert abs(loss - loss_r) <= 1e-4 * loss_r
     for name, off, shape_ in param_layout():
         size = int(np.prod(shape_))
         a, b = g[off:off + size].astype(np.float64), gr[off:off + size].astype(np.float64)
@@ -69,7 +72,7 @@ def test_gradients_match_cpu_autograd():
         cos = float(a @ b / (np.linalg.norm(a) * nb + 1e-30))
         rel = float(np.linalg.norm(a - b) / (nb + 1e-30))
         _print(f"{name:7s} |g| {nb:.3e} cos {cos:.5f} rel {rel:.3e}")
-        assert cos >= 0.99 and rel <= 0.06, name
+        assert cos >= 0.9999 and rel <= 6e-3, name
 
 
 def test_trained_weights_beat_the_noisy_input():

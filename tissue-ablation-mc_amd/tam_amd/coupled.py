@@ -50,25 +50,35 @@ def run_form(form, prob, cycles, steps, hi, lo, tol=0.02, dn=None, evaluate=Fals
     t_eval = 0.0                      # evaluation-only work and logging, not timed
     for c in range(cycles):
         cyc = dict(cycle=c)
+        split = {}
+        t_ph = time.perf_counter()
         if form == "plain":
             st = g.mc_run(hi, seed=seed, stream=c)
             npk = hi
+            split["transport"] = 1e3 * (time.perf_counter() - t_ph)
         else:
             if evaluate:
                 t0 = time.perf_counter()
                 g.mc_run(hi, seed=seed, stream=5000 + c)
                 ref = g.download(lib.FIELD_TALLY) / (hi * vcm)
                 t_eval += time.perf_counter() - t0
+            t_ph = time.perf_counter()
             g.mc_run(lo, seed=seed, stream=3000 + c)
             B = g.tally_blocks(BLOCK)
+            split["independent_run_and_blocks"] = 1e3 * (time.perf_counter() - t_ph)
             if evaluate:
                 t0 = time.perf_counter()
                 ind = g.download(lib.FIELD_TALLY)
                 t_eval += time.perf_counter() - t0
+            t_ph = time.perf_counter()
             st = g.mc_run(lo, seed=seed, stream=c)
+            t1 = time.perf_counter()
             dn.apply(g, lo)
             A = g.tally_blocks(BLOCK)
+            t2 = time.perf_counter()
             resid = block_residual(A, B)
+            split.update(transport=1e3 * (t1 - t_ph), denoise_and_blocks=1e3 * (t2 - t1),
+                         residual=1e3 * (time.perf_counter() - t2), forward_ms=dn.last_ms())
             cyc["residual"] = resid
             npk = lo
             if evaluate:
@@ -77,10 +87,13 @@ def run_form(form, prob, cycles, steps, hi, lo, tol=0.02, dn=None, evaluate=Fals
                 phi_d, phi_i = den / (lo * vcm), ind / (lo * vcm)
                 cyc["fluence_error_denoised"] = learned.fluence_error(phi_d, ref)
                 cyc["fluence_error_plain_lo"] = learned.fluence_error(phi_i, ref)
+                cyc["fluence_bias_denoised"] = learned.fluence_bias(phi_d, ref)
                 t_eval += time.perf_counter() - t0
             if resid > tol:
                 rec["fallbacks"] += 1
+                t_ph = time.perf_counter()
                 st = g.mc_run(hi, seed=seed, stream=c)
+                split["fallback"] = 1e3 * (time.perf_counter() - t_ph)
                 npk = hi
                 cyc["fallback"] = True
         g.normalise(prob.power / (npk * p.volume_voxel))
@@ -101,7 +114,7 @@ def run_form(form, prob, cycles, steps, hi, lo, tol=0.02, dn=None, evaluate=Fals
         t0 = time.perf_counter()
         m = g.download(lib.FIELD_MAT)
         tm = g.timing()
-        cyc.update(npk=npk, removed=int((m == len(prob.table)).sum()), heat_ms=tm.heat_ms,
+        cyc.update(npk=npk, removed=int((m == len(prob.table)).sum()), heat_ms=tm.heat_ms, split_ms=split,
                    heat_shape=[int(tm.tile), int(tm.kz)], mc_ms=st.kernel_ms)
         rec["cycles"].append(cyc)
         if log:
@@ -138,6 +151,9 @@ def compare(prob, plain, chosen, ev=None):
     if ev is not None:
         e = [c["fluence_error_denoised"][0] for c in ev["cycles"]]
         e1 = [c["fluence_error_plain_lo"][0] for c in ev["cycles"]]
+        b = [c["fluence_bias_denoised"] for c in ev["cycles"]]
+        out.update(median_fluence_bias_denoised=float(np.median([x[0] for x in b])),
+                   sum_fluence_bias_denoised=float(np.median([x[1] for x in b])))
         out.update(median_fluence_error_denoised=float(np.median(e)),
                    max_cycle_fluence_error_denoised=float(np.max(e)),
                    median_fluence_error_plain_lo=float(np.median(e1)))

@@ -35,7 +35,9 @@ def random_layers(rng):
 
 
 def layered_problem(n, layers, crater=None, beam_w=0.15, ext=EXT):
-    """Problem on an n^3 grid; crater = (radius_cm, depth_cm) or None."""
+    """Problem on an n^3 grid; crater = (radius_cm, depth_cm) for a cylinder,
+    (radius_cm, depth_cm, "bowl") for a paraboloid of that radius and
+    central depth, or None."""
     p = configs._config_params((n, n, n), ext)
     p.beam = lib.BEAM_GAUSSIAN
     p.beam_radius = beam_w
@@ -47,10 +49,12 @@ def layered_problem(n, layers, crater=None, beam_w=0.15, ext=EXT):
     rk = np.zeros((n + 2,) * 3)
     rk[1:-1, 1:-1, 1:-1] = mut[mat]
     if crater is not None:
-        r, d = crater
+        r, d = crater[:2]
         zz, yy, xx = np.meshgrid((np.arange(n) + 0.5) * dz, (np.arange(n) + 0.5) * ext[1] / n,
                                  (np.arange(n) + 0.5) * ext[0] / n, indexing="ij")
-        inside = ((xx - ext[0] / 2) ** 2 + (yy - ext[1] / 2) ** 2 <= r * r) & (zz >= ext[2] - d)
+        rho2 = (xx - ext[0] / 2) ** 2 + (yy - ext[1] / 2) ** 2
+        depth = d * (1.0 - rho2 / (r * r)) if len(crater) > 2 and crater[2] == "bowl" else d
+        inside = (rho2 <= r * r) & (zz >= ext[2] - depth)
         mat[inside] = len(table) - 1
         rk[1:-1, 1:-1, 1:-1][inside] = 0.0
     temp, kappa, density, heatcap = configs._thermal_cube(p, configs.C0_THERMAL)
@@ -65,13 +69,37 @@ def random_problem(n, rng):
     return layered_problem(n, layers, crater), layers, crater
 
 
-def make_pairs(dn, slots, n, seed, npk_hi, npk_lo_choices=(100_000, 250_000), log=print):
-    """Fill dataset slots with MC pairs; returns per-slot metadata."""
+def random_skin_layers(rng):
+    """Three-layer skin around the coupled workload's (epidermis 0.01 cm,
+    20 / 470 / 0.79; dermis 0.2 cm, 2.7 / 200 / 0.82; subcutis 1.0 / 130 /
+    0.8; n = 1.4), each property jittered: the epidermis lies outside the
+    1-4-layer draws (mu_a <= 10, mu_s <= 300 /cm)."""
+    u = lambda lo, hi: float(rng.uniform(lo, hi))
+    lg = lambda lo, hi: float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+    return [(u(0.005, 0.02), lg(10.0, 30.0), lg(300.0, 600.0), u(0.75, 0.85), 1.4),
+            (u(0.1, 0.3), lg(1.5, 4.0), lg(150.0, 250.0), u(0.78, 0.86), 1.4),
+            (1e9, lg(0.5, 2.0), lg(100.0, 160.0), u(0.75, 0.85), 1.4)]
+
+
+def random_skin_problem(n, rng):
+    layers = random_skin_layers(rng)
+    crater = None
+    if rng.random() < 0.7:                       # most coupled cycles run on a crater
+        crater = (float(rng.uniform(0.03, 0.25)), float(rng.uniform(0.01, 0.2)))
+        if rng.random() < 0.5:                   # ablation under a Gaussian beam digs a bowl
+            crater = crater + ("bowl",)
+    return layered_problem(n, layers, crater), layers, crater
+
+
+def make_pairs(dn, slots, n, seed, npk_hi, npk_lo_choices=(100_000, 250_000), log=print, kind="layers"):
+    """Fill dataset slots with MC pairs; returns per-slot metadata.  kind:
+    "layers" (1-4 random planar layers) or "skin" (jittered three-layer skin)."""
     rng = np.random.default_rng(seed)
     meta = []
+    draw = random_skin_problem if kind == "skin" else random_problem
     for s in slots:
         t0 = time.perf_counter()
-        prob, layers, crater = random_problem(n, rng)
+        prob, layers, crater = draw(n, rng)
         npk_lo = int(rng.choice(npk_lo_choices))
         lo, hi = prob.grid(), prob.grid()
         st_lo = lo.mc_run(npk_lo, seed=2 * (seed * 100003 + s) + 1)
@@ -79,7 +107,7 @@ def make_pairs(dn, slots, n, seed, npk_hi, npk_lo_choices=(100_000, 250_000), lo
         dn.store(s, lo, npk_lo, hi, npk_hi)
         lo.close()
         hi.close()
-        meta.append(dict(slot=s, layers=layers, crater=crater, npk_lo=npk_lo, npk_hi=npk_hi,
+        meta.append(dict(slot=s, kind=kind, layers=layers, crater=crater, npk_lo=npk_lo, npk_hi=npk_hi,
                          mc_ms=st_hi.kernel_ms + st_lo.kernel_ms, interactions=st_hi.interactions / npk_hi,
                          wall_s=time.perf_counter() - t0))
         log(f"pair {s}: {len(layers)} layers, crater {crater}, lo {npk_lo}, hi {npk_hi:.0e}, "
@@ -103,3 +131,10 @@ def fluence_error(phi, ref, frac=1e-2):
     sel = ref > frac * ref.max()
     e = np.abs(phi[sel] - ref[sel]) / ref[sel]
     return float(np.median(e)), float(np.percentile(e, 90)), int(sel.sum())
+
+
+def fluence_bias(phi, ref, frac=1e-2):
+    """Signed bias of phi against ref over voxels with ref > frac * max(ref):
+    the median of (phi - ref) / ref, and sum(phi) / sum(ref) - 1."""
+    sel = ref > frac * ref.max()
+    return float(np.median((phi[sel] - ref[sel]) / ref[sel])), float(phi[sel].sum() / ref[sel].sum() - 1.0)
