@@ -250,6 +250,13 @@ def c4_leg(args, log):
     ev = coupled.run_form("chosen-eval", prob, dn=dn, evaluate=True, **kw)
     dn.close()
     rep = coupled.compare(prob, plain, chosen, ev)
+    # the noise floor of the accuracy figures: a second plain run on other
+    # Philox streams, compared with the first the same way (untimed)
+    floor = None
+    if not args.c4_no_floor:
+        plain2 = coupled.run_form("plain", prob, seed=12, tune=False, **kw)
+        floor = coupled.compare(prob, plain, plain2)
+        log(f"c4 plain, second seed: peak T difference {floor['peak_T_error']:.3f} K")
     # per-cycle time split: transport (+ denoiser) vs the stencil both forms run
     heat_p = sum(c["heat_ms"] for c in plain["cycles"]) / 1e3
     heat_c = sum(c["heat_ms"] for c in chosen["cycles"]) / 1e3
@@ -272,8 +279,22 @@ def c4_leg(args, log):
                             "peak_T_le_0.5K": rep["peak_T_error"] <= 0.5},
            "split_s": {"plain_transport": m_p, "plain_stencil": heat_p, "chosen_transport_denoise": m_c,
                        "chosen_stencil": heat_c}}
-    # 20x needs (m_p + S) / (m_c + S) >= 20 with S the stencil time both run
+    # 20x needs (m_p + S) / (m_c + S) >= 20 with S the stencil time both run:
+    # S <= (m_p - 20 m_c) / 19, negative when m_p / m_c < 20 (no stencil speed reaches it)
     out["stencil_bound_for_20x_s"] = (m_p - 20.0 * m_c) / 19.0
+    out["ratio_with_zero_stencil_time"] = m_p / m_c
+    split = [c.get("split_ms", {}) for c in chosen["cycles"]]
+    out["chosen_ms_per_cycle"] = {k: float(np.mean([q.get(k, 0.0) for q in split]))
+                                  for k in ("independent_run_and_blocks", "transport", "denoise_and_blocks",
+                                            "residual", "forward_ms")}
+    out["chosen_ms_per_cycle"]["stencil"] = 1e3 * heat_c / len(chosen["cycles"])
+    out["plain_ms_per_cycle"] = {"transport": 1e3 * m_p / len(plain["cycles"]),
+                                 "stencil": 1e3 * heat_p / len(plain["cycles"])}
+    if floor is not None:
+        out["noise_floor_plain_vs_plain"] = {
+            "peak_T_error_K": floor["peak_T_error"], "p99_abs_T_diff_tissue_K": floor["p99_abs_T_diff_tissue"],
+            "max_abs_T_diff_tissue_K": floor["max_abs_T_diff_tissue"], "crater_rel_diff": floor["crater_rel_diff"],
+            "basis": "two plain 1e8-packet runs on different Philox streams"}
     return out
 
 
@@ -415,6 +436,7 @@ def main():
     ap.add_argument("--c4-grid", type=int, default=256)
     ap.add_argument("--c4-cycles", type=int, default=20)
     ap.add_argument("--c4-steps", type=int, default=5000)
+    ap.add_argument("--c4-no-floor", action="store_true", help="skip the second plain run (accuracy noise floor)")
     ap.add_argument("--leg", default="main", choices=("main", "stencil-state", "transport-state", "denoiser"))
     ap.add_argument("--state", nargs=2, metavar=("save|run", "PATH"), default=None)
     ap.add_argument("--shapes", default="4:8", help="stencil-state leg: comma list of tile:kz")
