@@ -50,9 +50,11 @@ def test_forward_matches_cpu_reference(shape):
     assert np.median(err) <= 5e-4 * span
 
 
-def test_gradients_match_cpu_autograd():
-    shape = (32, 32, 32)
-    _print("case: denoiser gradients 32^3")
+# (160, 64, 64): more than 1024 output bricks at full resolution, so the
+# brick weight-gradient kernels loop over several bricks per block
+@pytest.mark.parametrize("shape", [(32, 32, 32), (160, 64, 64)])
+def test_gradients_match_cpu_autograd(shape):
+    _print(f"case: denoiser gradients {shape}")
     dn = Denoiser(shape, capacity=1, seed=1)
     p = R.random_params(dn.n, seed=5)
     dn.set_params(p)
@@ -76,7 +78,7 @@ def test_gradients_match_cpu_autograd():
 
 
 def test_trained_weights_beat_the_noisy_input():
-    """The committed weights (trained on 3000 128^3 pairs) on a fresh layered
+    """The committed weights (trained on 2000 128^3 pairs) on a fresh layered
     tissue they never saw: 1e5-packet input vs a 2e7-packet reference.  The
     denoised fluence's median per-voxel error (voxels above 1% of peak) is
     below a third of the plain low-count MC's and below 5%."""

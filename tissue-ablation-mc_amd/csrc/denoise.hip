@@ -471,6 +471,214 @@ __global__ __launch_bounds__(256) void wgrad3_kernel(WgradArgs a)
     }
 }
 
+// Weight gradient by output bricks, all of a 16-channel input tile's taps at
+// once: dW^T[co][ci] += sum_m dy[m][co] x[m + tap][ci].
+//
+// A block stages a 16 (x) x 8 (y) x 4 (z) brick of dy (COT x 16 channels) and
+// the matching 18 x 10 x 6 brick of x (one 16-channel tile, zeros outside the
+// grid), both transposed to [channel][voxel] in LDS, once per brick.  Wave w
+// owns plane w of the brick: four 32-voxel k-chunks (row pairs), whose dy
+// fragments stay in registers while it walks the taps.  Per chunk and (dz, dy)
+// one aligned 8-voxel x read plus two dwords give the dx = -1, 0, +1 fragments
+// (v_alignbyte), so one LDS read feeds 3 x COT MFMAs.  NDZ = 1: the block
+// covers all 27 taps; NDZ = 3: the 9 taps of one dz (blockIdx.y % 3).
+// blockIdx.y / NDZ is the input-channel tile.  Blocks loop over bricks and
+// keep their sums in registers; at the end the four waves are summed in LDS
+// and each block writes its partial [taps][16][CO] to `part`, which
+// wgradb_reduce_kernel sums into gw.  The blocks of grid row 0 also sum dy
+// over their bricks (the bias gradient, CO floats after the weights).
+struct WgradbArgs {
+    const uint16_t* x;      // layer input [nvox][cin]
+    const uint16_t* dy;     // output gradient [nvox][cout]
+    float* part;            // [gridDim.y][gridDim.x]: [taps][16][cout] weights, then [cout] bias
+    int D, H, W, cin, cout;
+    long bricks;            // (W / 16) (H / 8) (D / 4)
+};
+
+template <int COT, int NDZ>
+__global__ __launch_bounds__(256) void wgradb_kernel(WgradbArgs a)
+{
+    constexpr int CO = COT * 16, NT = 27 / NDZ;
+    constexpr int XR = 24;                                  // x row: element 4 + v holds voxel x0 + v (v = -2 .. 17)
+    constexpr int XC = 6 * 10 * XR + 4;                     // x channel stride (elements), padded against bank conflicts
+    constexpr int DC = 4 * 8 * 16 + 8;                      // dy channel stride (elements), padded likewise
+    constexpr int NW = NT * 16 * CO;                       // weights per partial
+    constexpr int XBYTES = 16 * XC * 2, DBYTES = CO * DC * 2, RBYTES = (NW + CO) * 4;
+    constexpr int LBYTES = (XBYTES + DBYTES > RBYTES ? XBYTES + DBYTES : RBYTES);
+    __shared__ __attribute__((aligned(16))) unsigned char lds[LBYTES];
+    uint16_t* sX = (uint16_t*)lds;
+    uint16_t* sD = (uint16_t*)(lds + XBYTES);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r16 = lane & 15, g = lane >> 4, h = g & 1, gr = g >> 1;
+    const int ct = (int)blockIdx.y / NDZ;                   // input-channel tile
+    const int dz0 = NDZ == 1 ? -1 : (int)blockIdx.y % 3 - 1;
+    const int W = a.W, H = a.H, D = a.D;
+    const int bx = W / 16, by = H / 8;
+    f32x4 acc[NT][COT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int i = 0; i < COT; ++i) acc[t][i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bool bias = blockIdx.y == 0;
+    float bsum[COT];
+#pragma unroll
+    for (int i = 0; i < COT; ++i) bsum[i] = 0.f;
+
+    for (long b = blockIdx.x; b < a.bricks; b += gridDim.x) {
+        const int x0 = (int)(b % bx) * 16, y0 = (int)((b / bx) % by) * 8, z0 = (int)(b / ((long)bx * by)) * 4;
+        __syncthreads();                                    // the last brick's reads are done
+        // x: voxel pairs (x0 + 2p, x0 + 2p + 1), p = -1 .. 8, of 10 rows x 6
+        // planes, 16 channels as two 8-channel halves; each lane loads both
+        // voxels' half and writes one dword (the pair) per channel.  Voxels
+        // x0 - 2 and x0 + 17 come along; their halves are never used.
+        for (int q = threadIdx.x; q < 10 * 10 * 6 * 2; q += 256) {
+            const int vq = q >> 1, half = q & 1;
+            const int pp = vq % 10, ly = (vq / 10) % 10, lz = vq / 100;
+            const int xx = x0 + 2 * pp - 2, yy = y0 - 1 + ly, zz = z0 - 1 + lz;
+            uint4 v0 = make_uint4(0u, 0u, 0u, 0u), v1 = make_uint4(0u, 0u, 0u, 0u);
+            if (yy >= 0 && yy < H && zz >= 0 && zz < D) {
+                const uint16_t* src = a.x + (((long)zz * H + yy) * W) * a.cin + ct * 16 + half * 8;
+                if (xx >= 0 && xx < W) v0 = *(const uint4*)(src + (long)xx * a.cin);
+                if (xx + 1 >= 0 && xx + 1 < W) v1 = *(const uint4*)(src + (long)(xx + 1) * a.cin);
+            }
+            const uint16_t* e0 = (const uint16_t*)&v0;
+            const uint16_t* e1 = (const uint16_t*)&v1;
+            uint16_t* dst = sX + (half * 8) * XC + (lz * 10 + ly) * XR + 2 + 2 * pp;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) *(uint32_t*)(dst + j * XC) = (uint32_t)e0[j] | ((uint32_t)e1[j] << 16);
+        }
+        // dy: voxel pairs of 16 x 8 x 4 voxels, CO channels in 8-channel groups
+        for (int q = threadIdx.x; q < 256 * (CO / 8); q += 256) {
+            const int vp = q / (CO / 8), c8 = (q % (CO / 8)) * 8;
+            const int vq = 2 * vp;
+            const int lx = vq & 15, ly = (vq >> 4) & 7, lz = vq >> 7;
+            const uint16_t* src = a.dy + (((long)(z0 + lz) * H + (y0 + ly)) * W + (x0 + lx)) * a.cout + c8;
+            const uint4 v0 = *(const uint4*)src;
+            const uint4 v1 = *(const uint4*)(src + a.cout);
+            const uint16_t* e0 = (const uint16_t*)&v0;
+            const uint16_t* e1 = (const uint16_t*)&v1;
+            uint16_t* dst = sD + c8 * DC + vq;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) *(uint32_t*)(dst + j * DC) = (uint32_t)e0[j] | ((uint32_t)e1[j] << 16);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            // dy^T fragment of chunk c: channel r16 of tile i, voxels 8h .. 8h + 7 of row 2c + gr, plane wv
+            bf16x8 ad[COT];
+#pragma unroll
+            for (int i = 0; i < COT; ++i) ad[i] = load8(sD + (i * 16 + r16) * DC + (wv * 8 + 2 * c + gr) * 16 + 8 * h);
+            if (bias) {
+#pragma unroll
+                for (int i = 0; i < COT; ++i)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) bsum[i] += (float)ad[i][j];
+            }
+            // all of the chunk's x reads first (one wait for the lot), then its MFMAs
+            constexpr int NR = (NDZ == 1 ? 3 : 1) * 3;
+            uint2 m0[NR], m1[NR];
+            unsigned prev[NR], next[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                // x row of channel r16 at plane wv + dz, row 2c + gr + dy (halo coordinates +1)
+                const int dz = dz0 + r / 3, ky = r % 3;
+                const uint16_t* row = sX + r16 * XC + ((wv + dz + 1) * 10 + (2 * c + gr + ky)) * XR;
+                m0[r] = *(const uint2*)(row + 4 + 8 * h);                       // voxels 8h .. 8h + 3
+                m1[r] = *(const uint2*)(row + 8 + 8 * h);                       // voxels 8h + 4 .. 8h + 7
+                prev[r] = *(const unsigned*)(row + 2 + 8 * h);                  // high half: voxel 8h - 1
+                next[r] = *(const unsigned*)(row + 12 + 8 * h);                 // low half: voxel 8h + 8
+            }
+#pragma unroll
+            for (int kz = 0; kz < (NDZ == 1 ? 3 : 1); ++kz) {
+#pragma unroll
+                for (int ky = 0; ky < 3; ++ky) {
+                    const int r = kz * 3 + ky;
+                    uint4 mid, lo, hi;
+                    mid.x = m0[r].x; mid.y = m0[r].y; mid.z = m1[r].x; mid.w = m1[r].y;
+                    lo.x = __builtin_amdgcn_alignbyte(mid.x, prev[r], 2);
+                    lo.y = __builtin_amdgcn_alignbyte(mid.y, mid.x, 2);
+                    lo.z = __builtin_amdgcn_alignbyte(mid.z, mid.y, 2);
+                    lo.w = __builtin_amdgcn_alignbyte(mid.w, mid.z, 2);
+                    hi.x = __builtin_amdgcn_alignbyte(mid.y, mid.x, 2);
+                    hi.y = __builtin_amdgcn_alignbyte(mid.z, mid.y, 2);
+                    hi.z = __builtin_amdgcn_alignbyte(mid.w, mid.z, 2);
+                    hi.w = __builtin_amdgcn_alignbyte(next[r], mid.w, 2);
+                    const bf16x8 bx3[3] = {__builtin_bit_cast(bf16x8, lo), __builtin_bit_cast(bf16x8, mid),
+                                           __builtin_bit_cast(bf16x8, hi)};
+#pragma unroll
+                    for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+                        for (int i = 0; i < COT; ++i) {
+                            const int t = (kz * 3 + ky) * 3 + kx;
+                            acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ad[i], bx3[kx], acc[t][i], 0, 0, 0);
+                        }
+                }
+            }
+        }
+    }
+    // the four waves' sums: wave 0 stores, the others add in turn; then the
+    // block's partial goes out (element (t, ci, co) at (t * 16 + ci) * CO + co)
+    float* sR = (float*)lds;
+    __syncthreads();                                        // the last brick's reads are done
+    for (int w = 0; w < 4; ++w) {
+        if (wv == w) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int i = 0; i < COT; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int q = (t * 16 + r16) * CO + i * 16 + 4 * g + j;
+                        sR[q] = (w == 0 ? 0.f : sR[q]) + acc[t][i][j];
+                    }
+        }
+        __syncthreads();
+    }
+    // bias: the four k-groups of a channel, then the waves in turn
+#pragma unroll
+    for (int i = 0; i < COT; ++i) {
+        bsum[i] += __shfl_xor(bsum[i], 16, 64);
+        bsum[i] += __shfl_xor(bsum[i], 32, 64);
+    }
+    for (int w = 0; w < 4; ++w) {
+        if (wv == w && g == 0) {
+#pragma unroll
+            for (int i = 0; i < COT; ++i) sR[NW + i * 16 + r16] = (w == 0 ? 0.f : sR[NW + i * 16 + r16]) + bsum[i];
+        }
+        __syncthreads();
+    }
+    float* out = a.part + ((long)blockIdx.y * gridDim.x + blockIdx.x) * (NW + CO);
+    for (int q = threadIdx.x; q < NW + CO; q += 256) out[q] = sR[q];
+}
+
+// gw[tap][ci][co] += the sum over the nb blocks of one blockIdx.y of their
+// partials; ny rows of blocks, ndz dz groups per input-channel tile.  Block
+// column blockIdx.y of this grid sums partials k = blockIdx.y, blockIdx.y +
+// gridDim.y, ... and adds its share atomically, so a weight sums over enough
+// blocks in parallel to hide the memory latency.
+__global__ __launch_bounds__(256) void wgradb_reduce_kernel(const float* __restrict__ part, int nb, int ny, int ndz,
+                                                            int cin, int cout, float* __restrict__ gw,
+                                                            float* __restrict__ gb)
+{
+    const int nt = 27 / ndz;
+    const long nw = (long)nt * 16 * cout, per = nw + cout;
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= per * ny) return;
+    const int yrow = (int)(q / per);
+    const long e = q % per;
+    if (e >= nw && yrow != 0) return;                       // only grid row 0 sums the bias
+    const float* src = part + (long)yrow * nb * per + e;
+    float s = 0.f;
+    for (int k = (int)blockIdx.y; k < nb; k += (int)gridDim.y) s += src[(long)k * per];
+    if (e >= nw) {
+        atomicAdd(&gb[e - nw], s);
+        return;
+    }
+    const int t = (int)(e / (16 * cout)), ci = (int)((e / cout) % 16), co = (int)(e % cout);
+    const int ct = yrow / ndz, tap = (ndz == 1 ? 0 : (yrow % 3) * 9) + t;
+    atomicAdd(&gw[((long)tap * cin + ct * 16 + ci) * cout + co], s);
+}
+
 // bias gradient: gb[c] += sum_m d[m][c], c a multiple of 8 (<= 2048); each lane reads 8 channels of a voxel
 __global__ __launch_bounds__(256) void bgrad_kernel(const uint16_t* d, long nvox, int c, float* gb)
 {
@@ -824,6 +1032,8 @@ struct tam_dn {
     uint16_t *x0, *a[NL], *p1, *p3, *c6, *c8;
     uint16_t *d[NL], *gc6, *gc8, *gp1, *gp3;
     float *out, *feat;                // FP32 head output; FP32 feature staging (5 x nvox)
+    float* wpart;                     // per-block weight-gradient partials of the brick kernel
+    long wpart_n;                     // its size in floats
     double* red;                      // [0] loss, [1] input loss
     // dataset
     int capacity;
@@ -861,7 +1071,7 @@ hipError_t conv_launch(tam_dn* n, const uint16_t* x, int lvl, int cin, const uin
     hipLaunchKernelGGL((conv_kernel<CI, NT_, MT_>), dim3(MT_ == 8 ? b8 : b4), dim3(256), 0, n->stream, a)
     const int D = n->D[lvl], H = n->H[lvl], W = n->W[lvl];
     if (W % 16 == 0 && H % 8 == 0 && D % 4 == 0) {
-        // LDS-staged input for the layers with 16 or 32 output channels
+        // LDS-staged input for the layers with 16, 32 or 48 output channels
         const dim3 grid((unsigned)((W / 16) * (H / 8) * (D / 4)));
 #define CL(CI, NT_) hipLaunchKernelGGL((conv_lds_kernel<CI, NT_>), grid, dim3(256), 0, n->stream, a)
         switch (cin * 1000 + cout) {
@@ -871,6 +1081,7 @@ hipError_t conv_launch(tam_dn* n, const uint16_t* x, int lvl, int cin, const uin
         case 96032: CL(96, 2); return hipGetLastError();
         case 48016: CL(48, 1); return hipGetLastError();
         case 32016: CL(32, 1); return hipGetLastError();
+        case 16048: CL(16, 3); return hipGetLastError();       // data gradient of layer 8
         }
 #undef CL
     }
@@ -892,8 +1103,53 @@ hipError_t conv_launch(tam_dn* n, const uint16_t* x, int lvl, int cin, const uin
     return hipGetLastError();
 }
 
-hipError_t wgrad_launch(tam_dn* n, const uint16_t* x, const uint16_t* dy, int lvl, int cin, int cout, float* gw)
+// The brick weight-gradient launch of a layer: grid rows (input-channel
+// tiles x dz groups), dz groups, blocks per row and partial floats per block;
+// false where the layer keeps the segment kernels (grid edges not multiples of
+// 16 / 8 / 4, or 64 output channels).
+struct BrickPlan {
+    int ny, ndz;
+    long nb, per;
+};
+bool brick_plan(int D, int H, int W, int cin, int cout, BrickPlan* bp)
 {
+    if (W % 16 != 0 || H % 8 != 0 || D % 4 != 0 || cin % 16 != 0) return false;
+    if (cout != 16 && cout != 32) return false;
+    bp->ndz = (cout == 16 && cin <= 48) ? 1 : 3;           // all 27 taps' sums in registers when they fit
+    bp->ny = (cin / 16) * bp->ndz;
+    const long bricks = (long)(W / 16) * (H / 8) * (D / 4);
+    long nb = 1024 / bp->ny;
+    if (nb < 1) nb = 1;
+    bp->nb = nb < bricks ? nb : bricks;
+    bp->per = (27 / bp->ndz) * 16L * cout + cout;
+    return true;
+}
+
+// weight gradient of one layer into gw; the brick form also sums the bias
+// gradient into gb and says so in *bias_done
+hipError_t wgrad_launch(tam_dn* n, const uint16_t* x, const uint16_t* dy, int lvl, int cin, int cout, float* gw,
+                        float* gb, bool* bias_done)
+{
+    *bias_done = false;
+    BrickPlan bp;
+    if (brick_plan(n->D[lvl], n->H[lvl], n->W[lvl], cin, cout, &bp) && n->wpart &&
+        (long)bp.ny * bp.nb * bp.per <= n->wpart_n) {
+        WgradbArgs b{x, dy, n->wpart, n->D[lvl], n->H[lvl], n->W[lvl], cin, cout,
+                     (long)(n->W[lvl] / 16) * (n->H[lvl] / 8) * (n->D[lvl] / 4)};
+        const dim3 grid((unsigned)bp.nb, (unsigned)bp.ny);
+        if (cout == 16 && bp.ndz == 1) hipLaunchKernelGGL((wgradb_kernel<1, 1>), grid, dim3(256), 0, n->stream, b);
+        else if (cout == 16) hipLaunchKernelGGL((wgradb_kernel<1, 3>), grid, dim3(256), 0, n->stream, b);
+        else hipLaunchKernelGGL((wgradb_kernel<2, 3>), grid, dim3(256), 0, n->stream, b);
+        // ~1024 reduction blocks: each weight's nb partials split over ksplit of them
+        const unsigned qb = grid_of(bp.ny * bp.per);
+        long ks = 1024 / (long)qb;
+        if (ks < 1) ks = 1;
+        if (ks > bp.nb) ks = bp.nb;
+        hipLaunchKernelGGL(wgradb_reduce_kernel, dim3(qb, (unsigned)ks), dim3(256), 0, n->stream, n->wpart,
+                           (int)bp.nb, bp.ny, bp.ndz, cin, cout, gw, gb);
+        *bias_done = true;
+        return hipGetLastError();
+    }
     WgradArgs a{x, dy, gw, n->D[lvl], n->H[lvl], n->W[lvl], cin, cout, 0, 0};
     a.blocks = n->nv[lvl] / 32;
     // rows of whole 32-voxel segments: one workgroup per (dz, dy) covers the three dx taps, ~288 per (dz, dy);
@@ -983,8 +1239,9 @@ hipError_t backward(tam_dn* n, const float* tgt)
                        n->p + n->head_off, n->nv[0], n->d[9], n->g + n->head_off);
     auto grads = [&](int l, const uint16_t* in, const uint16_t* dl) {
         const Layer& L = n->L[l];
-        hipError_t r = wgrad_launch(n, in, dl, L.level, L.cin, L.cout, n->g + L.w_off);
-        if (r != hipSuccess) return r;
+        bool bias_done = false;
+        hipError_t r = wgrad_launch(n, in, dl, L.level, L.cin, L.cout, n->g + L.w_off, n->g + L.b_off, &bias_done);
+        if (r != hipSuccess || bias_done) return r;
         // at most 1024 blocks: every block ends in `cout` atomics onto the same few addresses, which serialise
         if (L.cout % 8 != 0 || L.cout > 2048) return hipErrorInvalidValue;
         unsigned nb = grid_of(n->nv[L.level], 256 / (L.cout / 8));
@@ -1139,6 +1396,14 @@ int tam_dn_create(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t ca
     al((void**)&n->out, 4ull * n->nv[0]);
     al((void**)&n->feat, 4ull * 5 * n->nv[0]);
     al((void**)&n->red, 16);
+    n->wpart_n = 0;
+    for (int l = 0; l < NL; ++l) {
+        const Layer& L = n->L[l];
+        BrickPlan bp;
+        if (brick_plan(n->D[L.level], n->H[L.level], n->W[L.level], L.cin, L.cout, &bp))
+            n->wpart_n = std::max(n->wpart_n, (long)bp.ny * bp.nb * bp.per);
+    }
+    if (n->wpart_n > 0) al((void**)&n->wpart, sizeof(float) * n->wpart_n);
     al((void**)&n->ds_feat, 8ull * n->nv[0] * (capacity > 0 ? capacity : 1));
     al((void**)&n->ds_tgt, 4ull * n->nv[0] * (capacity > 0 ? capacity : 1));
     if (e != hipSuccess) {
@@ -1180,7 +1445,7 @@ void tam_dn_destroy(tam_dn* n)
     DevScope ds(n->dev);
     if (n->stream) hipStreamSynchronize(n->stream);
     std::vector<void*> ptrs = {n->p, n->g, n->m1, n->m2, n->x0, n->p1, n->p3, n->c6, n->c8, n->gc6, n->gc8,
-                               n->gp1, n->gp3, n->out, n->feat, n->red, n->ds_feat, n->ds_tgt};
+                               n->gp1, n->gp3, n->out, n->feat, n->red, n->ds_feat, n->ds_tgt, n->wpart};
     for (int l = 0; l < NL; ++l) {
         ptrs.push_back(n->L[l].wf);
         ptrs.push_back(n->L[l].wb);
