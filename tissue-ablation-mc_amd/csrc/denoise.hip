@@ -204,35 +204,41 @@ __global__ __launch_bounds__(256) void conv_lds_kernel(ConvArgs a)
             *(bf16x8*)(sx + vq * 16 + half * 8) = v;
         }
         __syncthreads();
-        // not unrolled: with the tap offsets as constants the compiler keeps all
-        // 14 k-blocks' addresses live across the channel-group loop
-#pragma unroll 1
-        for (int kb = 0; kb < 14; ++kb) {
-            // lane group g: tap 2 kb + g / 2, channels 16 cg + 8 (g % 2) .. + 7
-            const int tap = 2 * kb + (g >> 1), ci0 = 8 * (g & 1);
-            bf16x8 af[8], bfr[NT];
-            if (tap < 27) {
-                const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-                const int base = ((w + dz + 1) * BY + (dy + 1)) * BX + (r + dx + 1);
+        // k-block kb: lane group g takes tap 2 kb + g / 2, channels 16 cg +
+        // 8 (g % 2) .. + 7.  Tap 27 (the last block's second half) is the
+        // packed weights' zero padding (or past the descriptor's end, which
+        // reads zeros): its A fragment repeats tap 26, a finite value times 0.
+        auto load = [&](int kb, bf16x8* af, bf16x8* bfr) {
+            const int tapr = 2 * kb + (g >> 1), ci0 = 8 * (g & 1);
+            const int tap = tapr < 27 ? tapr : 26;
+            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+            const int base = ((w + dz + 1) * BY + (dy + 1)) * BX + (r + dx + 1);
 #pragma unroll
-                for (int t = 0; t < 8; ++t) af[t] = *(const bf16x8*)(sx + (base + t * BX) * 16 + ci0);
-                // this lane's k in the packed (tap-major) order: its k-block and lane group there
-                const int ko = tap * CIN + 16 * cg + ci0;
-                const int kbo = ko >> 5, go = (ko >> 3) & 3;
+            for (int t = 0; t < 8; ++t) af[t] = *(const bf16x8*)(sx + (base + t * BX) * 16 + ci0);
+            // this lane's k in the packed (tap-major) order: its k-block and lane group there
+            const int ko = tapr * CIN + 16 * cg + ci0;
+            const int kbo = ko >> 5, go = (ko >> 3) & 3;
 #pragma unroll
-                for (int n = 0; n < NT; ++n)
-                    bfr[n] = bload8(rw, (unsigned)((go * 16 + r) * 16), (unsigned)((kbo * NT + n) * 1024));
-            } else {
-#pragma unroll
-                for (int t = 0; t < 8; ++t) af[t] = bf16x8{};
-#pragma unroll
-                for (int n = 0; n < NT; ++n) bfr[n] = bf16x8{};
-            }
+            for (int n = 0; n < NT; ++n)
+                bfr[n] = bload8(rw, (unsigned)((go * 16 + r) * 16), (unsigned)((kbo * NT + n) * 1024));
+        };
+        auto mma = [&](const bf16x8* af, const bf16x8* bfr) {
 #pragma unroll
             for (int t = 0; t < 8; ++t)
 #pragma unroll
                 for (int n = 0; n < NT; ++n)
                     acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[t], bfr[n], acc[t][n], 0, 0, 0);
+        };
+        // two fragment sets in turn: the next k-block's reads are in flight
+        // while the current one's MFMAs run
+        bf16x8 afa[8], bfa[NT], afb[8], bfb[NT];
+        load(0, afa, bfa);
+#pragma unroll 1
+        for (int kb = 0; kb < 14; kb += 2) {
+            load(kb + 1, afb, bfb);
+            mma(afa, bfa);
+            if (kb + 2 < 14) load(kb + 2, afa, bfa);
+            mma(afb, bfb);
         }
     }
     // lane holds voxels x0 + 4 (lane / 16) + j of row (y0 + t, z0 + w), channel 16 n + lane % 16
@@ -1071,7 +1077,8 @@ hipError_t conv_launch(tam_dn* n, const uint16_t* x, int lvl, int cin, const uin
     hipLaunchKernelGGL((conv_kernel<CI, NT_, MT_>), dim3(MT_ == 8 ? b8 : b4), dim3(256), 0, n->stream, a)
     const int D = n->D[lvl], H = n->H[lvl], W = n->W[lvl];
     if (W % 16 == 0 && H % 8 == 0 && D % 4 == 0) {
-        // LDS-staged input for the layers with 16, 32 or 48 output channels
+        // LDS-staged input for the layers with 16 or 32 output channels (48
+        // outputs as NT = 3 needs ~260 VGPRs there: the global form is faster)
         const dim3 grid((unsigned)((W / 16) * (H / 8) * (D / 4)));
 #define CL(CI, NT_) hipLaunchKernelGGL((conv_lds_kernel<CI, NT_>), grid, dim3(256), 0, n->stream, a)
         switch (cin * 1000 + cout) {
@@ -1081,7 +1088,6 @@ hipError_t conv_launch(tam_dn* n, const uint16_t* x, int lvl, int cin, const uin
         case 96032: CL(96, 2); return hipGetLastError();
         case 48016: CL(48, 1); return hipGetLastError();
         case 32016: CL(32, 1); return hipGetLastError();
-        case 16048: CL(16, 3); return hipGetLastError();       // data gradient of layer 8
         }
 #undef CL
     }
