@@ -63,6 +63,7 @@ __global__ __launch_bounds__(256) void interp_kernel(const double* __restrict__ 
 }
 
 unsigned blocks_for(long n) { return (unsigned)((n + 255) / 256); }
+constexpr int64_t kMaxDraws = (int64_t)0x7fffffff * 256;    // one lane per element, at most 2^31 - 1 blocks
 
 // device buffers of one call, freed on every path out
 struct Bufs {
@@ -137,7 +138,8 @@ int tam_spectrum_sample(int32_t device, const double* table, const double* cdf, 
     // lin_inter_1D reads row nlow+2 <= len: at least three rows
     int rc = check_table(table, len, 3);
     if (rc != TAM_OK) return rc;
-    if (!cdf || (n > 0 && !wave) || n < 0) return tam_fail_noctx(TAM_ERR_ARG, "bad cdf, output or count");
+    if (!cdf || (n > 0 && !wave) || n < 0 || n > kMaxDraws)
+        return tam_fail_noctx(TAM_ERR_ARG, "bad cdf, output or count");
     for (int32_t j = 1; j < len - 1; ++j)
         if (cdf[j] < cdf[j - 1]) return tam_fail_noctx(TAM_ERR_ARG, "cdf must not decrease");
     if (n == 0) return TAM_OK;
@@ -165,7 +167,8 @@ int tam_spectrum_interp(int32_t device, const double* table, int32_t len, const 
 {
     int rc = check_table(table, len, 2);
     if (rc != TAM_OK) return rc;
-    if ((n > 0 && (!x || !y)) || n < 0) return tam_fail_noctx(TAM_ERR_ARG, "bad input, output or count");
+    if ((n > 0 && (!x || !y)) || n < 0 || n > kMaxDraws)
+        return tam_fail_noctx(TAM_ERR_ARG, "bad input, output or count");
     if (n == 0) return TAM_OK;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return tam_fail_noctx(TAM_ERR_NODEVICE, "no HIP device");
