@@ -40,13 +40,27 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t* p) { return __builtin_bi
 __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u)); }
 
 struct ConvArgs {
-    const uint16_t* x;      // input [nvox][cin]
+    const uint16_t* x;      // input [nvox][cin]; with x2: the coarse grid [nvox / 8][c1]
     const uint16_t* wp;     // packed weights [kb][nt][64][8]
     const float* bias;      // [cout] or nullptr
     const uint16_t* mask;   // [nvox][cout] or nullptr: output *= (mask > 0)
     uint16_t* y;            // output [nvox][cout]
     int D, H, W, cin, cout, kb, relu;
+    // input = cat(upsample2(x), x2) when x2 != nullptr: channels < c1 from the
+    // coarse grid (nearest), the other cin - c1 from x2 [nvox][cin - c1]
+    const uint16_t* x2;
+    int c1;
 };
+
+// 8 channels (c0 .. c0 + 7, c0 a multiple of 8) of fine voxel (xx, yy, zz):
+// from x, or with x2 from cat(upsample2(x), x2)
+__device__ __forceinline__ const uint16_t* cat_src(const uint16_t* x, const uint16_t* x2, int c1, int cin, int D,
+                                                   int H, int W, int xx, int yy, int zz, int c0)
+{
+    if (!x2) return x + (((long)zz * H + yy) * W + xx) * cin + c0;
+    if (c0 < c1) return x + (((long)(zz >> 1) * (H >> 1) + (yy >> 1)) * (W >> 1) + (xx >> 1)) * c1 + c0;
+    return x2 + (((long)zz * H + yy) * W + xx) * (cin - c1) + (c0 - c1);
+}
 
 // Raw buffer descriptor over `bytes` bytes from p (at most 2^31 - 1): loads
 // at offsets past it return zero.  0x00020000: gfx9 buffer word 3 (32-bit
@@ -200,7 +214,7 @@ __global__ __launch_bounds__(256) void conv_lds_kernel(ConvArgs a)
             const int xx = x0 - 1 + lx, yy = y0 - 1 + ly, zz = z0 - 1 + lz;
             bf16x8 v = {};
             if (xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D)
-                v = *(const bf16x8*)(a.x + (((long)zz * H + yy) * W + xx) * CIN + 16 * cg + half * 8);
+                v = *(const bf16x8*)cat_src(a.x, a.x2, a.c1, CIN, D, H, W, xx, yy, zz, 16 * cg + half * 8);
             *(bf16x8*)(sx + vq * 16 + half * 8) = v;
         }
         __syncthreads();
@@ -494,11 +508,13 @@ __global__ __launch_bounds__(256) void wgrad3_kernel(WgradArgs a)
 // wgradb_reduce_kernel sums into gw.  The blocks of grid row 0 also sum dy
 // over their bricks (the bias gradient, CO floats after the weights).
 struct WgradbArgs {
-    const uint16_t* x;      // layer input [nvox][cin]
+    const uint16_t* x;      // layer input [nvox][cin]; with x2: the coarse grid [nvox / 8][c1]
     const uint16_t* dy;     // output gradient [nvox][cout]
     float* part;            // [gridDim.y][gridDim.x]: [taps][16][cout] weights, then [cout] bias
     int D, H, W, cin, cout;
     long bricks;            // (W / 16) (H / 8) (D / 4)
+    const uint16_t* x2;     // input = cat(upsample2(x), x2) when not null (see ConvArgs)
+    int c1;
 };
 
 template <int COT, int NDZ>
@@ -543,9 +559,11 @@ __global__ __launch_bounds__(256) void wgradb_kernel(WgradbArgs a)
             const int xx = x0 + 2 * pp - 2, yy = y0 - 1 + ly, zz = z0 - 1 + lz;
             uint4 v0 = make_uint4(0u, 0u, 0u, 0u), v1 = make_uint4(0u, 0u, 0u, 0u);
             if (yy >= 0 && yy < H && zz >= 0 && zz < D) {
-                const uint16_t* src = a.x + (((long)zz * H + yy) * W) * a.cin + ct * 16 + half * 8;
-                if (xx >= 0 && xx < W) v0 = *(const uint4*)(src + (long)xx * a.cin);
-                if (xx + 1 >= 0 && xx + 1 < W) v1 = *(const uint4*)(src + (long)(xx + 1) * a.cin);
+                const int c0 = ct * 16 + half * 8;
+                if (xx >= 0 && xx < W)
+                    v0 = *(const uint4*)cat_src(a.x, a.x2, a.c1, a.cin, D, H, W, xx, yy, zz, c0);
+                if (xx + 1 >= 0 && xx + 1 < W)
+                    v1 = *(const uint4*)cat_src(a.x, a.x2, a.c1, a.cin, D, H, W, xx + 1, yy, zz, c0);
             }
             const uint16_t* e0 = (const uint16_t*)&v0;
             const uint16_t* e1 = (const uint16_t*)&v1;
@@ -740,27 +758,6 @@ __global__ void pool_kernel(const uint16_t* x, int D, int H, int W, int C, uint1
 #pragma unroll
         for (int j = 0; j < 8; ++j) o.h[j] = f2bf(0.125f * s[j]);
         st8(y + q * 8, o);
-    }
-}
-
-// cat(upsample2(coarse), skip): fine grid D x H x W, channels C1 + C2; one
-// thread per fine voxel and 8-channel chunk
-__global__ void upcat_kernel(const uint16_t* coarse, int C1, const uint16_t* skip, int C2, int D, int H, int W,
-                             uint16_t* y)
-{
-    const int C8 = (C1 + C2) / 8, C18 = C1 / 8;
-    const long n = (long)D * H * W * C8;
-    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const int c8 = (int)(q % C8);
-        const long m = q / C8;
-        if (c8 < C18) {
-            const int x = (int)(m % W);
-            const long r = m / W;
-            const int yy = (int)(r % H), z = (int)(r / H);
-            st8(y + q * 8, ld8(coarse + ((((long)(z / 2) * (H / 2)) + yy / 2) * (W / 2) + x / 2) * C1 + c8 * 8));
-        } else {
-            st8(y + q * 8, ld8(skip + m * C2 + (c8 - C18) * 8));
-        }
     }
 }
 
@@ -1035,7 +1032,7 @@ struct tam_dn {
     float *p, *g, *m1, *m2;           // parameters, gradients, Adam moments
     long step;
     // activations (BF16) and their gradients
-    uint16_t *x0, *a[NL], *p1, *p3, *c6, *c8;
+    uint16_t *x0, *a[NL], *p1, *p3;
     uint16_t *d[NL], *gc6, *gc8, *gp1, *gp3;
     float *out, *feat;                // FP32 head output; FP32 feature staging (5 x nvox)
     float* wpart;                     // per-block weight-gradient partials of the brick kernel
@@ -1064,9 +1061,10 @@ int dfail(tam_dn* n, int code, const std::string& m)
 thread_local std::string g_dn_err;
 
 hipError_t conv_launch(tam_dn* n, const uint16_t* x, int lvl, int cin, const uint16_t* wp, int kb, const float* bias,
-                       int cout, int relu, const uint16_t* mask, uint16_t* y)
+                       int cout, int relu, const uint16_t* mask, uint16_t* y, const uint16_t* x2 = nullptr,
+                       int c1 = 0)
 {
-    ConvArgs a{x, wp, bias, mask, y, n->D[lvl], n->H[lvl], n->W[lvl], cin, cout, kb, relu};
+    ConvArgs a{x, wp, bias, mask, y, n->D[lvl], n->H[lvl], n->W[lvl], cin, cout, kb, relu, x2, c1};
     const long nv = n->nv[lvl];
     // 16 x MT-voxel wave tiles: 64 voxels, or 128 for the 48-channel input
     // layer (its long k loop amortises the extra registers); every level's
@@ -1091,6 +1089,7 @@ hipError_t conv_launch(tam_dn* n, const uint16_t* x, int lvl, int cin, const uin
         }
 #undef CL
     }
+    if (x2) return hipErrorInvalidValue;  // a concatenated input is read by the LDS form only
     switch (cin * 1000 + cout) {
     case 16016: CV(16, 1, 4); break;      // layers 0, 1, 9 and the data gradients of 0, 1, 9
     case 16032: CV(16, 2, 4); break;      // layer 2
@@ -1134,14 +1133,14 @@ bool brick_plan(int D, int H, int W, int cin, int cout, BrickPlan* bp)
 // weight gradient of one layer into gw; the brick form also sums the bias
 // gradient into gb and says so in *bias_done
 hipError_t wgrad_launch(tam_dn* n, const uint16_t* x, const uint16_t* dy, int lvl, int cin, int cout, float* gw,
-                        float* gb, bool* bias_done)
+                        float* gb, bool* bias_done, const uint16_t* x2 = nullptr, int c1 = 0)
 {
     *bias_done = false;
     BrickPlan bp;
     if (brick_plan(n->D[lvl], n->H[lvl], n->W[lvl], cin, cout, &bp) && n->wpart &&
         (long)bp.ny * bp.nb * bp.per <= n->wpart_n) {
         WgradbArgs b{x, dy, n->wpart, n->D[lvl], n->H[lvl], n->W[lvl], cin, cout,
-                     (long)(n->W[lvl] / 16) * (n->H[lvl] / 8) * (n->D[lvl] / 4)};
+                     (long)(n->W[lvl] / 16) * (n->H[lvl] / 8) * (n->D[lvl] / 4), x2, c1};
         const dim3 grid((unsigned)bp.nb, (unsigned)bp.ny);
         if (cout == 16 && bp.ndz == 1) hipLaunchKernelGGL((wgradb_kernel<1, 1>), grid, dim3(256), 0, n->stream, b);
         else if (cout == 16) hipLaunchKernelGGL((wgradb_kernel<1, 3>), grid, dim3(256), 0, n->stream, b);
@@ -1156,6 +1155,7 @@ hipError_t wgrad_launch(tam_dn* n, const uint16_t* x, const uint16_t* dy, int lv
         *bias_done = true;
         return hipGetLastError();
     }
+    if (x2) return hipErrorInvalidValue;  // a concatenated input is read by the brick form only
     WgradArgs a{x, dy, gw, n->D[lvl], n->H[lvl], n->W[lvl], cin, cout, 0, 0};
     a.blocks = n->nv[lvl] / 32;
     // rows of whole 32-voxel segments: one workgroup per (dz, dy) covers the three dx taps, ~288 per (dz, dy);
@@ -1219,13 +1219,14 @@ hipError_t forward(tam_dn* n, const float* tgt)
                        n->W[1], 32, n->p3);
     if ((e = conv(4, n->p3, n->a[4])) != hipSuccess) return e;
     if ((e = conv(5, n->a[4], n->a[5])) != hipSuccess) return e;
-    hipLaunchKernelGGL(upcat_kernel, dim3(grid_of(n->nv[1] * 96 / 8)), dim3(256), 0, n->stream, n->a[5], 64, n->a[3], 32,
-                       n->D[1], n->H[1], n->W[1], n->c6);
-    if ((e = conv(6, n->c6, n->a[6])) != hipSuccess) return e;
+    // layers 6 and 8 read cat(upsample2(coarse), skip) in their input staging
+    auto conv_cat = [&](int l, const uint16_t* coarse, int c1, const uint16_t* skip, uint16_t* o) {
+        const Layer& L = n->L[l];
+        return conv_launch(n, coarse, L.level, L.cin, L.wf, L.kb_f, n->p + L.b_off, L.cout, 1, nullptr, o, skip, c1);
+    };
+    if ((e = conv_cat(6, n->a[5], 64, n->a[3], n->a[6])) != hipSuccess) return e;
     if ((e = conv(7, n->a[6], n->a[7])) != hipSuccess) return e;
-    hipLaunchKernelGGL(upcat_kernel, dim3(grid_of(n->nv[0] * 48 / 8)), dim3(256), 0, n->stream, n->a[7], 32, n->a[1], 16,
-                       n->D[0], n->H[0], n->W[0], n->c8);
-    if ((e = conv(8, n->c8, n->a[8])) != hipSuccess) return e;
+    if ((e = conv_cat(8, n->a[7], 32, n->a[1], n->a[8])) != hipSuccess) return e;
     if ((e = conv(9, n->a[8], n->a[9])) != hipSuccess) return e;
     if (tgt) hipLaunchKernelGGL(fill_f64k, dim3(1), dim3(64), 0, n->stream, n->red, 2L, 0.0);
     // the head kernels end every workgroup in atomics onto a few addresses: at most 1024 workgroups
@@ -1243,10 +1244,11 @@ hipError_t backward(tam_dn* n, const float* tgt)
     const unsigned hb = grid_of(n->nv[0]) < 1024u ? grid_of(n->nv[0]) : 1024u;
     hipLaunchKernelGGL(head_bwd_kernel, dim3(hb), dim3(256), 0, n->stream, n->out, tgt, n->a[9],
                        n->p + n->head_off, n->nv[0], n->d[9], n->g + n->head_off);
-    auto grads = [&](int l, const uint16_t* in, const uint16_t* dl) {
+    auto grads = [&](int l, const uint16_t* in, const uint16_t* dl, const uint16_t* x2 = nullptr, int c1 = 0) {
         const Layer& L = n->L[l];
         bool bias_done = false;
-        hipError_t r = wgrad_launch(n, in, dl, L.level, L.cin, L.cout, n->g + L.w_off, n->g + L.b_off, &bias_done);
+        hipError_t r = wgrad_launch(n, in, dl, L.level, L.cin, L.cout, n->g + L.w_off, n->g + L.b_off, &bias_done,
+                                    x2, c1);
         if (r != hipSuccess || bias_done) return r;
         // at most 1024 blocks: every block ends in `cout` atomics onto the same few addresses, which serialise
         if (L.cout % 8 != 0 || L.cout > 2048) return hipErrorInvalidValue;
@@ -1262,13 +1264,13 @@ hipError_t backward(tam_dn* n, const float* tgt)
     };
     if ((e = grads(9, n->a[8], n->d[9])) != hipSuccess) return e;
     if ((e = dgrad(9, n->d[9], n->a[8], n->d[8])) != hipSuccess) return e;
-    if ((e = grads(8, n->c8, n->d[8])) != hipSuccess) return e;
+    if ((e = grads(8, n->a[7], n->d[8], n->a[1], 32)) != hipSuccess) return e;
     if ((e = dgrad(8, n->d[8], nullptr, n->gc8)) != hipSuccess) return e;
     hipLaunchKernelGGL(upcat_bwd_kernel, dim3(grid_of(n->nv[1] * 32 / 8)), dim3(256), 0, n->stream, n->gc8, 48, 32,
                        n->a[7], n->D[0], n->H[0], n->W[0], n->d[7]);
     if ((e = grads(7, n->a[6], n->d[7])) != hipSuccess) return e;
     if ((e = dgrad(7, n->d[7], n->a[6], n->d[6])) != hipSuccess) return e;
-    if ((e = grads(6, n->c6, n->d[6])) != hipSuccess) return e;
+    if ((e = grads(6, n->a[5], n->d[6], n->a[3], 64)) != hipSuccess) return e;
     if ((e = dgrad(6, n->d[6], nullptr, n->gc6)) != hipSuccess) return e;
     hipLaunchKernelGGL(upcat_bwd_kernel, dim3(grid_of(n->nv[2] * 64 / 8)), dim3(256), 0, n->stream, n->gc6, 96, 64,
                        n->a[5], n->D[1], n->H[1], n->W[1], n->d[5]);
@@ -1393,8 +1395,6 @@ int tam_dn_create(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t ca
     al((void**)&n->x0, 2ull * n->nv[0] * 16);
     al((void**)&n->p1, 2ull * n->nv[1] * 16);
     al((void**)&n->p3, 2ull * n->nv[2] * 32);
-    al((void**)&n->c6, 2ull * n->nv[1] * 96);
-    al((void**)&n->c8, 2ull * n->nv[0] * 48);
     al((void**)&n->gc6, 2ull * n->nv[1] * 96);
     al((void**)&n->gc8, 2ull * n->nv[0] * 48);
     al((void**)&n->gp1, 2ull * n->nv[1] * 16);
@@ -1450,7 +1450,7 @@ void tam_dn_destroy(tam_dn* n)
     if (!n) return;
     DevScope ds(n->dev);
     if (n->stream) hipStreamSynchronize(n->stream);
-    std::vector<void*> ptrs = {n->p, n->g, n->m1, n->m2, n->x0, n->p1, n->p3, n->c6, n->c8, n->gc6, n->gc8,
+    std::vector<void*> ptrs = {n->p, n->g, n->m1, n->m2, n->x0, n->p1, n->p3, n->gc6, n->gc8,
                                n->gp1, n->gp3, n->out, n->feat, n->red, n->ds_feat, n->ds_tgt, n->wpart};
     for (int l = 0; l < NL; ++l) {
         ptrs.push_back(n->L[l].wf);
