@@ -856,6 +856,31 @@ __global__ void ctx_features_kernel(const double* tally, const float* mut, const
     }
 }
 
+// the same features written straight into the 16-channel BF16 input (the
+// forward-only path: no FP32 staging)
+__global__ void ctx_input_kernel(const double* tally, const float* mut, const uint8_t* mat, const MatEntry* mtab,
+                                 long nvox, double inv_nv, uint16_t* x0)
+{
+    for (long m = (long)blockIdx.x * blockDim.x + threadIdx.x; m < nvox; m += (long)gridDim.x * blockDim.x) {
+        const float mu = mut[m], alb = mtab[mat[m]].albedo;
+        const float mua = mu * (1.f - alb), mus = mu * alb;
+        uint16_t v[16];
+        // round to FP32 first, as the staged path stores it: the empty asm keeps
+        // the compiler from folding double -> float -> BF16 into one rounding
+        float l = (float)log1p(tally[m] * inv_nv);
+        asm volatile("" : "+v"(l));
+        v[0] = f2bf(l);
+        v[1] = f2bf(log10f(fmaxf(mua, 1e-3f)));
+        v[2] = f2bf(log10f(fmaxf(mus, 1e-2f)) - 2.f);
+        v[3] = f2bf(5.f * (mtab[mat[m]].g - 0.8f));
+#pragma unroll
+        for (int c = 4; c < 16; ++c) v[c] = 0;
+        uint4* o = (uint4*)(x0 + m * 16);
+        o[0] = *(uint4*)&v[0];
+        o[1] = *(uint4*)&v[8];
+    }
+}
+
 // features (4 x nvox FP32) + target log1p(phi_hi) from a context -> slot
 __global__ void store_slot_kernel(const float* f, const double* tally_hi, double inv_nv_hi, long nvox,
                                   uint16_t* feat, float* tgt)
@@ -1229,8 +1254,11 @@ hipError_t forward(tam_dn* n, const float* tgt)
     if ((e = conv_cat(8, n->a[7], 32, n->a[1], n->a[8])) != hipSuccess) return e;
     if ((e = conv(9, n->a[8], n->a[9])) != hipSuccess) return e;
     if (tgt) hipLaunchKernelGGL(fill_f64k, dim3(1), dim3(64), 0, n->stream, n->red, 2L, 0.0);
-    // the head kernels end every workgroup in atomics onto a few addresses: at most 1024 workgroups
-    const unsigned hb = grid_of(n->nv[0]) < 1024u ? grid_of(n->nv[0]) : 1024u;
+    // with a target every workgroup ends in atomics onto two addresses: at most
+    // 1024 workgroups then; without one, a workgroup per 256 voxels (up to
+    // grid_of's cap) so the loads of many voxels are in flight at once
+    const unsigned hg = grid_of(n->nv[0]);
+    const unsigned hb = tgt && hg > 1024u ? 1024u : hg;
     hipLaunchKernelGGL(head_kernel, dim3(hb), dim3(256), 0, n->stream, n->x0, n->a[9],
                        n->p + n->head_off, n->nv[0], n->out, tgt, n->red, n->red + 1);
     return hipGetLastError();
@@ -1662,9 +1690,8 @@ int tam_dn_apply(tam_dn* n, tam_ctx* c, double npk)
     const double vcm = c->p.volume_voxel * 1e6;
     DNCHK(n, hipStreamSynchronize(c->stream));
     DNCHK(n, hipEventRecord(n->ev0, n->stream));
-    hipLaunchKernelGGL(ctx_features_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream, c->tally, c->mut, c->mat,
-                       c->mtab, nv, 1.0 / (npk * vcm), n->feat);
-    hipLaunchKernelGGL(pack_input_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream, n->feat, nv, n->x0);
+    hipLaunchKernelGGL(ctx_input_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream, c->tally, c->mut, c->mat,
+                       c->mtab, nv, 1.0 / (npk * vcm), n->x0);
     DNCHK(n, forward(n, nullptr));
     hipLaunchKernelGGL(write_tally_kernel, dim3(grid_of(nv)), dim3(256), 0, n->stream, n->out, c->mut, nv, npk * vcm,
                        c->tally);
