@@ -987,7 +987,7 @@ __global__ void adam_kernel(float* p, const float* g, float* m1, float* m2, long
 // element (kb, nt, lane, j) = W at k = 32 kb + 8 (lane/16) + j, co = 16 nt +
 // lane%16.  flip: the data-gradient weights W'[26 - tap][co][ci] (cin and
 // cout swap roles).
-__global__ void pack_w_kernel(const float* w, int cin, int cout, int flip, int kb_n, uint16_t* wp)
+__device__ __forceinline__ void pack_w(const float* w, int cin, int cout, int flip, int kb_n, uint16_t* wp)
 {
     const int ci_eff = flip ? cout : cin, co_eff = flip ? cin : cout;
     const int nt = co_eff / 16;
@@ -1004,6 +1004,23 @@ __global__ void pack_w_kernel(const float* w, int cin, int cout, int flip, int k
         if (tap < 27) v = flip ? w[((long)(26 - tap) * cin + co) * cout + ci] : w[((long)tap * cin + ci) * cout + co];
         wp[q] = f2bf(v);
     }
+}
+
+// every layer's forward and data-gradient packing in one launch: grid row
+// blockIdx.y packs job y (rows past a small job's extent exit at once)
+struct PackJob {
+    const float* w;
+    uint16_t* wp;
+    int cin, cout, flip, kb_n;
+};
+constexpr int kPackJobs = 20;           // two per layer (the network's ten layers)
+struct PackJobs {
+    PackJob j[kPackJobs];
+};
+__global__ void pack_all_kernel(PackJobs jobs)
+{
+    const PackJob& J = jobs.j[blockIdx.y];
+    pack_w(J.w, J.cin, J.cout, J.flip, J.kb_n, J.wp);
 }
 
 __global__ void fill_f32k(float* p, long n, float v)
@@ -1039,6 +1056,7 @@ struct Layer {
 };
 
 constexpr int NL = 10;
+static_assert(2 * NL == kPackJobs, "one forward and one data-gradient packing per layer");
 constexpr int CIN[NL] = {16, 16, 16, 32, 32, 64, 96, 32, 48, 16};
 constexpr int COUT[NL] = {16, 16, 32, 32, 64, 64, 32, 32, 16, 16};
 constexpr int LEVEL[NL] = {0, 0, 1, 1, 2, 2, 1, 1, 0, 0};
@@ -1136,7 +1154,7 @@ hipError_t conv_launch(tam_dn* n, const uint16_t* x, int lvl, int cin, const uin
 // The brick weight-gradient launch of a layer: grid rows (input-channel
 // tiles x dz groups), dz groups, blocks per row and partial floats per block;
 // false where the layer keeps the segment kernels (grid edges not multiples of
-// 16 / 8 / 4, or 64 output channels).
+// 16 / 8 / 4).
 struct BrickPlan {
     int ny, ndz;
     long nb, per;
@@ -1144,7 +1162,7 @@ struct BrickPlan {
 bool brick_plan(int D, int H, int W, int cin, int cout, BrickPlan* bp)
 {
     if (W % 16 != 0 || H % 8 != 0 || D % 4 != 0 || cin % 16 != 0) return false;
-    if (cout != 16 && cout != 32) return false;
+    if (cout != 16 && cout != 32 && cout != 64) return false;
     bp->ndz = (cout == 16 && cin <= 48) ? 1 : 3;           // all 27 taps' sums in registers when they fit
     bp->ny = (cin / 16) * bp->ndz;
     const long bricks = (long)(W / 16) * (H / 8) * (D / 4);
@@ -1169,7 +1187,8 @@ hipError_t wgrad_launch(tam_dn* n, const uint16_t* x, const uint16_t* dy, int lv
         const dim3 grid((unsigned)bp.nb, (unsigned)bp.ny);
         if (cout == 16 && bp.ndz == 1) hipLaunchKernelGGL((wgradb_kernel<1, 1>), grid, dim3(256), 0, n->stream, b);
         else if (cout == 16) hipLaunchKernelGGL((wgradb_kernel<1, 3>), grid, dim3(256), 0, n->stream, b);
-        else hipLaunchKernelGGL((wgradb_kernel<2, 3>), grid, dim3(256), 0, n->stream, b);
+        else if (cout == 32) hipLaunchKernelGGL((wgradb_kernel<2, 3>), grid, dim3(256), 0, n->stream, b);
+        else hipLaunchKernelGGL((wgradb_kernel<4, 3>), grid, dim3(256), 0, n->stream, b);
         // ~1024 reduction blocks: each weight's nb partials split over ksplit of them
         const unsigned qb = grid_of(bp.ny * bp.per);
         long ks = 1024 / (long)qb;
@@ -1215,14 +1234,17 @@ hipError_t wgrad_launch(tam_dn* n, const uint16_t* x, const uint16_t* dy, int lv
 
 hipError_t repack(tam_dn* n)
 {
+    PackJobs jobs;
+    unsigned gx = 1;
     for (int l = 0; l < NL; ++l) {
-        Layer& L = n->L[l];
+        const Layer& L = n->L[l];
         const float* w = n->p + L.w_off;
-        hipLaunchKernelGGL(pack_w_kernel, dim3(grid_of((long)L.kb_f * L.cout * 32)), dim3(256), 0, n->stream, w,
-                           L.cin, L.cout, 0, L.kb_f, L.wf);
-        hipLaunchKernelGGL(pack_w_kernel, dim3(grid_of((long)L.kb_b * L.cin * 32)), dim3(256), 0, n->stream, w,
-                           L.cin, L.cout, 1, L.kb_b, L.wb);
+        jobs.j[2 * l] = PackJob{w, L.wf, L.cin, L.cout, 0, L.kb_f};
+        jobs.j[2 * l + 1] = PackJob{w, L.wb, L.cin, L.cout, 1, L.kb_b};
+        gx = std::max(gx, grid_of((long)L.kb_f * L.cout * 32));
+        gx = std::max(gx, grid_of((long)L.kb_b * L.cin * 32));
     }
+    hipLaunchKernelGGL(pack_all_kernel, dim3(gx, kPackJobs), dim3(256), 0, n->stream, jobs);
     return hipGetLastError();
 }
 
