@@ -1,8 +1,9 @@
 """Batched instances (tam_batch_*) against the same instances run one at a
 time: the transport of every instance in one launch gives each instance the
 integer fates of its own tam_mc_run on the same stream and the same tally up
-to FP64 summation order; batched heat steps are bit-identical with the
-single-instance step of the same launch shape."""
+to FP64 summation order (the batched kernel's per-wave LDS boxes hold FP64
+sums, so the single runs use the FP64 box); batched heat steps are
+bit-identical with the single-instance step of the same launch shape."""
 import numpy as np
 import pytest
 
@@ -37,6 +38,7 @@ def test_batch_transport_matches_single_runs():
     b.close()
     for i, g in enumerate(gs):
         g.set_mc_launch(2, 24)
+        g.set_mc_box(True, fp64=True)
         s1 = g.mc_run(npk[i], seed=42, stream=10 + i)
         t1 = g.download(lib.FIELD_TALLY)
         for k in ("absorbed", "specular", "missed", "crossings", "interactions", "tally_adds"):

@@ -108,29 +108,80 @@ def test_mc_same_stream_through_a_crater():
 
 def test_mc_launch_shape_does_not_change_results():
     """Packets draw from their own streams whichever lane takes them: every
-    (blocks per CU, random-phase batch, LDS tally box on / off; with the box on,
-    4 and 8 run as 1 and 2 blocks of 1024 lanes) gives the same
-    integer fates and the same tally up to FP64 summation order.  The tuner
-    leaves the tally clear."""
+    (blocks per CU, random-phase batch, LDS tally box off / fixed point / FP64;
+    with the box on, 4 and 8 run as 1 and 2 blocks of 1024 lanes) gives the
+    same integer fates and the same tally up to FP64 summation order, or, with
+    the fixed-point box, up to the rounding of each deposit to its unit u =
+    2^-22 (dx + dy + dz): half a unit per deposit at most, ~1e-6 of a
+    voxel-sized path.  Bound used: 1e-5 relative plus 64 u (the worst case of
+    128 deposits, for voxels that took only a few short paths).  The tuner
+    keeps the box's kind and leaves the tally clear."""
     prob = _two_layer_problem(n=32, packets=300_000)
+    p = prob.params
+    unit = 2.0 * (p.xmax / p.nx + p.ymax / p.ny + p.zmax / p.nz) / 2 ** 22
     g = prob.grid()
     res = []
-    for bpc, batch, box in ((1, 1, 1), (2, 64, 1), (8, 32, 0), (5, 13, 1), (2, 32, 0), (4, 32, 1), (8, 16, 1)):
-        _print(f"case: launch bpc={bpc} batch={batch} box={box}")
+    for bpc, batch, box in ((1, 1, 2), (2, 64, 2), (8, 32, 0), (5, 13, 1), (2, 32, 0), (4, 32, 1), (8, 16, 1),
+                            (4, 32, 2)):
+        _print(f"case: launch bpc={bpc} batch={batch} box={box} (0 off, 1 fixed point, 2 FP64)")
         g.set_mc_launch(bpc, batch)
-        g.set_mc_box(box)
+        g.set_mc_box(box > 0, fp64=box == 2)
         st = g.mc_run(prob.packets, seed=77, stream=3)
-        res.append((st, g.download(lib.FIELD_TALLY)))
+        res.append((box, st, g.download(lib.FIELD_TALLY)))
     b, t, rate = g.tune_mc(100_000)
     assert 1 <= b <= 64 and 1 <= t <= 64 and rate > 0
     assert not g.download(lib.FIELD_TALLY).any()
     g.close()
-    st0, t0 = res[0]
-    for st, tl in res[1:]:
+    _, st0, t0 = res[0]
+    for box, st, tl in res[1:]:
         for k in ("absorbed", "specular", "missed", "crossings", "interactions", "killed"):
             assert getattr(st, k) == getattr(st0, k), k
         assert list(st.escaped) == list(st0.escaped)
-        np.testing.assert_allclose(tl, t0, rtol=1e-9, atol=1e-12)
+        if box == 1:
+            np.testing.assert_allclose(tl, t0, rtol=1e-5, atol=64 * unit)
+        else:
+            np.testing.assert_allclose(tl, t0, rtol=1e-9, atol=1e-12)
+
+
+def test_fixed_point_box_matches_fp64_box_at_c1_scale():
+    """The fixed-point LDS tally box against the FP64 one on the three-layer
+    skin at 200^3 with 1e8 packets (the transport of one c1 cycle), with the
+    box placed from a previous run: the same packets and fates, and tallies
+    that differ only by the rounding of each deposit to the box's unit (2^-22
+    of dx + dy + dz; at most half a unit per deposit, unbiased), ~1e-7 of a
+    voxel's sum.  The hot voxels here take ~1e4 box deposits per block, so a
+    slot passes 2^30 units and hands them to the tally many times.  Asserted:
+    1e-5 relative on every voxel above 1e-3 of the peak, 1e-6 on the total.
+    Two fixed-point runs give the same tally up to FP64 summation order: the
+    integer sums do not depend on the order of the adds."""
+    prob = configs.config1(n=(200, 200, 200), packets=100_000_000, steps=1, cycles=1)
+    g = prob.grid()
+    g.set_mc_launch(4, 32)
+    res = []
+    for fp64 in (True, False, False):
+        _print(f"case: 200^3 skin, 1e8 packets, {'FP64' if fp64 else 'fixed-point'} box")
+        g.set_mc_box(True, fp64=True)
+        g.mc_run(prob.packets, seed=98, stream=0)          # the path each box is placed from
+        g.set_mc_box(True, fp64=fp64)
+        st = g.mc_run(prob.packets, seed=99, stream=1)
+        res.append((st, g.download(lib.FIELD_TALLY), g.mc_box()))
+    g.close()
+    _print("boxes:", [r[2] for r in res])
+    assert res[1][2] == res[2][2]                         # both fixed-point runs placed the same box
+    res = [(st, t) for st, t, _ in res]
+    (s64, t64), (sfx, tfx), (sfx2, tfx2) = res
+    for k in ("absorbed", "specular", "missed", "crossings", "interactions", "killed", "tally_adds"):
+        assert getattr(sfx, k) == getattr(s64, k), k
+    _print("box adds per packet: FP64", s64.box_adds / prob.packets, "fixed point", sfx.box_adds / prob.packets)
+    assert sfx.box_adds > 1.1 * s64.box_adds                # the 32-bit box holds twice the voxels
+    hot = t64 > 1e-3 * t64.max()
+    rel = np.abs(tfx[hot] - t64[hot]) / t64[hot]
+    tot = abs(tfx.sum() / t64.sum() - 1)
+    _print(f"voxels compared {int(hot.sum())}: max rel {rel.max():.3e}, median {np.median(rel):.3e}; "
+           f"total rel {tot:.3e}")
+    assert rel.max() < 1e-5
+    assert tot < 1e-6
+    np.testing.assert_allclose(tfx2, tfx, rtol=1e-12, atol=0)
 
 
 def test_mc_box_follows_the_deposition_below_a_crater():
@@ -156,8 +207,8 @@ def test_mc_box_follows_the_deposition_below_a_crater():
     assert jm[: n - 24].sum() > 0.3 * jm.sum()          # much of the path below the crater floor
     res = []
     for box in (1, 0, 1):
-        _print(f"case: 40^3 crater 24 planes deep, box {box} placed from the previous run's path")
-        g.set_mc_box(box)
+        _print(f"case: 40^3 crater 24 planes deep, FP64 box {box} placed from the previous run's path")
+        g.set_mc_box(box, fp64=True)
         st = g.mc_run(prob.packets, seed=4321)
         res.append((st, g.download(lib.FIELD_TALLY)))
     g.close()
@@ -191,7 +242,7 @@ def test_mc_box_placement_is_the_best_window():
     prob.rhokap = rk
     g = prob.grid()
     g.set_mc_launch(4, 32)
-    g.set_mc_box(1)
+    g.set_mc_box(1, fp64=True)                                 # 27^3: the top-face window can move down
     g.mc_run(prob.packets, seed=5, stream=0)
     (x0, y0, z0), (sx, sy, sz) = g.mc_box()
     _print("fresh grid: box", (x0, y0, z0), (sx, sy, sz))

@@ -43,6 +43,9 @@ static __device__ __align__(256) double scratch[TAM_BLOCKSUM_SCRATCH_MAX];
 namespace {
 
 constexpr int kMaxDevices = 64;
+// tam_tune_mc keeps the CU-wide LDS tally box when it times within this
+// fraction of the fastest candidate
+constexpr double kWideBoxMargin = 0.10;
 std::mutex scratch_mu;
 double* scratch_dev[kMaxDevices] = {};
 
@@ -1250,8 +1253,9 @@ int tam_set_mc_launch(tam_ctx* c, int32_t blocks_per_cu, int32_t rng_batch)
 int tam_set_mc_box(tam_ctx* c, int32_t enable)
 {
     if (!c) return TAM_ERR_ARG;
+    if (enable < 0 || enable > 2) return fail(c, TAM_ERR_ARG, "box mode must be 0 (off), 1 (fixed point) or 2 (FP64)");
     cycle_reset(c);
-    c->mc_box = enable != 0;
+    c->mc_box = enable;
     return TAM_OK;
 }
 
@@ -1275,19 +1279,43 @@ int tam_tune_mc(tam_ctx* c, int64_t sample, int32_t* blocks_per_cu, int32_t* rng
     const int batches[] = {8, 16, 24, 32, 40, 48};
     double best = 0.0;
     int bb = c->mc_bpc, bt = c->mc_batch, bx = c->mc_box;
+    const int box_on = old_box ? old_box : 1;           // the box's kind is kept
+    struct Cand {
+        int bpc, batch, box;
+        double rate;
+    };
+    std::vector<Cand> rates;
     for (int x = 0; x < 2; ++x)
         for (int b : bpcs) {
             for (int t : batches) {
-                c->mc_box = x;
+                c->mc_box = x ? box_on : 0;
                 c->mc_bpc = b;
                 c->mc_batch = t;
                 tam_mc_stats st;
                 const int rc = tam_mc_run(c, sample, 0x5eed5eedULL, 0xffffffffu, 0, &st);
                 if (rc != TAM_OK) return rc;
                 const double rate = st.kernel_ms > 0 ? sample / (st.kernel_ms * 1e-3) : 0.0;
-                if (rate > best) { best = rate; bb = b; bt = t; bx = x; }
+                rates.push_back({b, t, c->mc_box, rate});
+                if (rate > best) { best = rate; bb = b; bt = t; bx = c->mc_box; }
             }
         }
+    // Ties go to the single CU-wide box (box on, 1024-lane blocks, one per
+    // CU): candidates are timed on the tissue as it is, and once a crater
+    // forms and the deposition moves below it, the largest box holds its lead
+    // (c1 crater phase: 360 ms per 1e8 packets against 465 ms for three
+    // boxes per CU, which can be a few percent ahead on fresh tissue).
+    if (!(bx && bb == 4)) {
+        double wide = 0.0;
+        int wt = bt;
+        for (size_t q = 0; q < rates.size(); ++q)
+            if (rates[q].box && rates[q].bpc == 4 && rates[q].rate > wide) { wide = rates[q].rate; wt = rates[q].batch; }
+        if (wide >= (1.0 - kWideBoxMargin) * best) {
+            bb = 4;
+            bt = wt;
+            bx = box_on;
+            best = wide;
+        }
+    }
     c->mc_bpc = bb;
     c->mc_batch = bt;
     c->mc_box = bx;

@@ -38,7 +38,12 @@
 //    block therefore sums the visits to a cube of voxels on the beam axis in
 //    LDS (mc_box_kernel), placed in z where the last run's path was largest
 //    (box_plane_kernel, box_window_kernel), and adds the cube to the tally
-//    once at exit.
+//    once at exit.  By default the box holds fixed-point sums in 32-bit
+//    integers (units of 2^-22 of dx + dy + dz): twice the voxels of an FP64
+//    box in the same LDS, integer LDS adds cost less than FP64 ones, and an
+//    integer sum does not depend on the order of the adds.  Each add that
+//    takes a slot past a multiple of 2^30 units moves 2^30 units to the
+//    device tally, so a slot never overflows.
 //  * Transcendentals use the hardware approximations (v_log, v_sin/v_cos
 //    in revolutions, v_sqrt, v_rcp/v_rsq; ~1 ulp): they only shape sampled
 //    distances and directions, whose distributions the parity tests check.
@@ -46,6 +51,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace {
 
@@ -95,6 +101,8 @@ __device__ __forceinline__ float face_dist(float o, float lo, float d, float u, 
 
 constexpr uint32_t kMaxEvents = 1u << 24;   // walk steps + random events of one packet
 constexpr uint32_t kChunk = 256;            // packets a wave takes from the queue at once
+constexpr uint32_t kBoxCarry = 1u << 30;    // a fixed-point box slot hands each 2^30 units to the tally
+constexpr float kBoxMaxAdd = 67108864.f;    // 2^26 units: a larger deposit goes to the tally directly
 
 enum : int { PEND_NONE = 0, PEND_LAUNCH = 1, PEND_INTERACT = 2, PEND_FRESNEL = 3 };
 
@@ -105,27 +113,69 @@ struct BoxGeom {
     int o[3], n[3];
 };
 
+// A fixed-point box add whose carry test is still to run: the slot (-1:
+// none), its value before the add, the add, and the tally voxel.
+struct BoxAdd {
+    int i;
+    uint32_t old, q, v;
+};
+
 // Path length into the tally: voxels inside the block's LDS box are summed
-// there (an LDS FP64 atomic), the rest go to the device tally directly.
-// Returns 1 for a deposit into the box.
-template <bool BOX>
-__device__ __forceinline__ uint32_t tally_add(const McArgs& a, double* box, const BoxGeom& g, uint32_t v, int ci,
-                                              int cj, int ck, float l)
+// there (BT = uint32_t: fixed point, BT = double: FP64), the rest go to the
+// FP64 device tally directly.  Returns 1 for a deposit into the box.  A
+// fixed-point add leaves its carry test in pa (box_carry), so that the LDS
+// return is awaited after the lane's other work.
+template <bool BOX, class BT>
+__device__ __forceinline__ uint32_t tally_add(const McArgs& a, BT* box, const BoxGeom& g, uint32_t v, int ci,
+                                              int cj, int ck, float l, BoxAdd& pa)
 {
     if (BOX) {
         const unsigned bx = (unsigned)(ci - g.o[0]), by = (unsigned)(cj - g.o[1]), bz = (unsigned)(ck - g.o[2]);
         if (bx < (unsigned)g.n[0] && by < (unsigned)g.n[1] && bz < (unsigned)g.n[2]) {
-            __hip_atomic_fetch_add(&box[bx + (unsigned)g.n[0] * (by + (unsigned)g.n[1] * bz)], (double)l,
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            return 1u;
+            const unsigned bi = bx + (unsigned)g.n[0] * (by + (unsigned)g.n[1] * bz);
+            if constexpr (std::is_same<BT, uint32_t>::value) {
+                const float qf = rintf(l * a.box_scale);
+                if (qf < kBoxMaxAdd) {
+                    pa.i = (int)bi;
+                    pa.q = (uint32_t)qf;
+                    pa.v = v;
+                    pa.old = __hip_atomic_fetch_add(&box[bi], pa.q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    return 1u;
+                }
+            } else {
+                __hip_atomic_fetch_add(&box[bi], (BT)l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                return 1u;
+            }
         }
     }
     unsafeAtomicAdd(&a.tally[v], (double)l);
     return 0u;
 }
 
-template <bool BOX>
-__device__ __forceinline__ void transport(const McArgs& a, double* box, const BoxGeom& g)
+// The carry test of a fixed-point box add: the add that takes its slot past a
+// multiple of kBoxCarry units moves kBoxCarry units from the slot to the
+// tally.  Every crossing is seen by exactly one add, each takes back what it
+// moved, so the slot plus the carries always equals the sum of the adds; a
+// slot stays below 2^32 while fewer than three carries of one slot are in
+// flight (each needs ~1000 adds to that slot).
+template <class BT>
+__device__ __forceinline__ void box_carry(const McArgs& a, BT* box, BoxAdd& pa)
+{
+    if constexpr (std::is_same<BT, uint32_t>::value) {
+        if (pa.i >= 0 && ((pa.old + pa.q) ^ pa.old) >= kBoxCarry) {
+            __hip_atomic_fetch_sub(&box[pa.i], kBoxCarry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            unsafeAtomicAdd(&a.tally[pa.v], (double)kBoxCarry * a.box_quantum);
+        }
+        pa.i = -1;
+    }
+}
+
+// A box slot's sum in path length
+__device__ __forceinline__ double box_path(uint32_t s, const McArgs& a) { return (double)s * a.box_quantum; }
+__device__ __forceinline__ double box_path(double s, const McArgs&) { return s; }
+
+template <bool BOX, class BT>
+__device__ __forceinline__ void transport(const McArgs& a, BT* box, const BoxGeom& g)
 {
     unsigned long long* head = a.counters + TAM_CNT_N;     // queue head, packets taken
     const uint32_t nxy = (uint32_t)a.nx * (uint32_t)a.ny;
@@ -137,6 +187,7 @@ __device__ __forceinline__ void transport(const McArgs& a, double* box, const Bo
 
     uint32_t c_abs = 0, c_spec = 0, c_miss = 0, c_cross = 0, c_inter = 0, c_kill = 0, c_tally = 0, c_box = 0;
     uint32_t c_esc[6] = {0, 0, 0, 0, 0, 0};
+    BoxAdd pa{-1, 0u, 0u, 0u};
 
     // wave-uniform queue state: packets [pool, pool_end) are this wave's
     uint64_t pool = 0, pool_end = 0;
@@ -247,7 +298,8 @@ __device__ __forceinline__ void transport(const McArgs& a, double* box, const Bo
                     ++c_inter;
                     const MatEntry me = a.mtab[mat];
                     if (!(u01(r.x) < me.albedo)) {
-                        c_box += tally_add<BOX>(a, box, g, v, ci, cj, ck, lacc);
+                        c_box += tally_add<BOX, BT>(a, box, g, v, ci, cj, ck, lacc, pa);
+                        box_carry<BT>(a, box, pa);
                         ++c_tally;
                         lacc = 0.f;
                         ++c_abs;
@@ -329,7 +381,7 @@ __device__ __forceinline__ void transport(const McArgs& a, double* box, const Bo
                 // mu_t = 0 (ablated): nothing is absorbed, and the atomic
                 // unit's throughput is what bounds this kernel
                 if (mu > 0.f) {
-                    c_box += tally_add<BOX>(a, box, g, v, ci, cj, ck, lacc);
+                    c_box += tally_add<BOX, BT>(a, box, g, v, ci, cj, ck, lacc, pa);
                     ++c_tally;
                 }
                 lacc = 0.f;
@@ -363,6 +415,7 @@ __device__ __forceinline__ void transport(const McArgs& a, double* box, const Bo
                 } else if (pend == PEND_NONE) {
                     mu = a.mut[v];
                 }
+                box_carry<BT>(a, box, pa);
             } else {
                 pend = PEND_INTERACT;
                 mat = a.mat[v];
@@ -384,16 +437,21 @@ __device__ __forceinline__ void transport(const McArgs& a, double* box, const Bo
     }
 }
 
-__global__ __launch_bounds__(256) void mc_kernel(McArgs a) { transport<false>(a, nullptr, BoxGeom{}); }
+__global__ __launch_bounds__(256) void mc_kernel(McArgs a)
+{
+    transport<false, double>(a, nullptr, BoxGeom{});
+}
 
 // The same with the block-private LDS tally box: cleared first, added to the
 // device tally (one atomic per touched voxel) once every wave of the block
 // has drained the queue.  NT = 256, or 1024: one block of 16 waves per CU
 // holds one box of the CU's whole LDS instead of four quarter-size copies.
-template <int NT>
+// BT = uint32_t (fixed point, the default box) or double.
+template <int NT, class BT>
 __global__ __launch_bounds__(NT) void mc_box_kernel(McArgs a)
 {
-    extern __shared__ double box[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char box_lds[];
+    BT* box = reinterpret_cast<BT*>(box_lds);
     // geometry: chosen by box_window_kernel (inside the grid, at most the
     // LDS this launch was given), or the launch's default
     BoxGeom g;
@@ -403,13 +461,13 @@ __global__ __launch_bounds__(NT) void mc_box_kernel(McArgs a)
         g.n[q] = a.box_geom ? __builtin_amdgcn_readfirstlane(a.box_geom[3 + q]) : a.box_n[q];
     }
     const int nbox = g.n[0] * g.n[1] * g.n[2];
-    for (int i = threadIdx.x; i < nbox; i += NT) box[i] = 0.0;
+    for (int i = threadIdx.x; i < nbox; i += NT) box[i] = (BT)0;
     __syncthreads();
-    transport<true>(a, box, g);
+    transport<true, BT>(a, box, g);
     __syncthreads();
     const uint32_t nxy = (uint32_t)a.nx * (uint32_t)a.ny;
     for (int i = threadIdx.x; i < nbox; i += NT) {
-        const double s = box[i];
+        const double s = box_path(box[i], a);
         if (s != 0.0) {
             const int bx = i % g.n[0], by = (i / g.n[0]) % g.n[1], bz = i / (g.n[0] * g.n[1]);
             const uint32_t v = (uint32_t)(g.o[0] + bx) + (uint32_t)a.nx * (uint32_t)(g.o[1] + by) +
@@ -533,13 +591,13 @@ __global__ __launch_bounds__(256) void mc_batch_kernel(const McArgs* __restrict_
         }
         const int nb = g.n[0] * g.n[1] * g.n[2];
         if (nb < 8) {
-            transport<false>(a, nullptr, g);
+            transport<false, double>(a, nullptr, g);
             continue;
         }
         for (int q = lane; q < nb; q += 64) box[q] = 0.0;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");   // the zeros land before any add
         __builtin_amdgcn_wave_barrier();
-        transport<true>(a, box, g);
+        transport<true, double>(a, box, g);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");   // every lane's adds land before the flush
         __builtin_amdgcn_wave_barrier();
         const uint32_t nxy = (uint32_t)a.nx * (uint32_t)a.ny;
@@ -615,10 +673,12 @@ hipError_t mc_prepare(int dev)
     hipError_t e = hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (e != hipSuccess || lds_blk <= 64 * 1024) return e;
     // allow the box kernels the device's whole per-block LDS (outside any graph capture)
-    e = hipFuncSetAttribute((const void*)mc_box_kernel<256>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_blk);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void*)mc_box_kernel<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_blk);
-    if (e != hipSuccess) return e;
+    const void* box_kernels[] = {(const void*)mc_box_kernel<256, uint32_t>, (const void*)mc_box_kernel<1024, uint32_t>,
+                                 (const void*)mc_box_kernel<256, double>, (const void*)mc_box_kernel<1024, double>};
+    for (const void* k : box_kernels) {
+        e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds_blk);
+        if (e != hipSuccess) return e;
+    }
     return hipFuncSetAttribute((const void*)mc_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_blk);
 }
 
@@ -650,17 +710,25 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
     if (need < blocks) blocks = need;
     // LDS tally box: the resident blocks of a CU share its LDS (160 KiB on
     // MI355X, also the most one block may take), 1 KiB per block kept for
-    // the kernel's own use: a cube of 27^3 voxels at 1 block per CU (1 or 4
-    // blocks_per_cu), 21^3 at 2 (2 or 8), 18^2 x 20 at 3, on the beam axis.  Its z range is where
+    // the kernel's own use: with 32-bit sums 34^2 x 35 voxels at 1 block per CU
+    // (1 or 4 blocks_per_cu), 27^3 at 2 (2 or 8), 23^3 at 3; with FP64 27^3,
+    // 21^3, 18^2 x 20.  The cube sits on the beam axis.  Its z range is where
     // the last run's path was largest, or the top face (where most of the
     // path of a surface beam lies) before any run.  On the c1 skin cycles a
     // cube caught more deposits past the first cycle than flatter (d ~ s/2,
     // s/8) or taller (d ~ 2s) boxes of the same volume, and than a per-launch
     // choice among depths of 2-32 planes made from the last run's path.
+    // The box holds fixed-point sums (box_enable 1: twice the voxels of FP64)
+    // or FP64 ones (box_enable 2).  Fixed-point unit: 2^-22 of dx + dy + dz,
+    // so a deposit's rounding is ~1e-6 of a typical path in a voxel at most.
+    const bool fx = a.box_enable == 1;
+    const long esz = fx ? 4 : 8;
     McArgs b = a;
+    b.box_scale = (float)std::ldexp(1.0 / ((double)a.dx + a.dy + a.dz), 22);
+    b.box_quantum = 1.0 / (double)b.box_scale;
     b.box_n[0] = b.box_n[1] = b.box_n[2] = 0;
     b.box_geom = nullptr;
-    const long vox = std::min((long)lds_blk - 1024L, (long)lds_cu / bpc_l - 1024L) / 8L;
+    const long vox = std::min((long)lds_blk - 1024L, (long)lds_cu / bpc_l - 1024L) / esz;
     BoxPlace bp{};
     bp.sd = std::min((int)std::cbrt((double)vox), std::min(a.nx, a.ny));
     bp.d = bp.sd > 0 ? (int)std::min<long>(a.nz, vox / ((long)bp.sd * bp.sd)) : 0;
@@ -672,7 +740,7 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
         b.box_o[2] = a.nz - bp.d;
         b.box_n[0] = b.box_n[1] = bp.sd;
         b.box_n[2] = bp.d;
-        const size_t lds = sizeof(double) * (size_t)bp.sd * (size_t)bp.sd * (size_t)bp.d;
+        const size_t lds = (size_t)esz * (size_t)bp.sd * (size_t)bp.sd * (size_t)bp.d;
         if (a.box_psum && a.box_geom) {
             hipLaunchKernelGGL(box_plane_kernel, dim3((unsigned)a.nz), dim3(256), 0, s, a.tally, a.mut, a.nx, a.ny,
                                bp, a.box_psum);
@@ -683,8 +751,13 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
             const hipError_t ef = launch_fill_f64(a.tally, nvox, 0.0, s);
             if (ef != hipSuccess) return ef;
         }
-        if (wide) hipLaunchKernelGGL(mc_box_kernel<1024>, dim3((unsigned)blocks), dim3(1024), lds, s, b);
-        else hipLaunchKernelGGL(mc_box_kernel<256>, dim3((unsigned)blocks), dim3(256), lds, s, b);
+        if (fx) {
+            if (wide) hipLaunchKernelGGL((mc_box_kernel<1024, uint32_t>), dim3((unsigned)blocks), dim3(1024), lds, s, b);
+            else hipLaunchKernelGGL((mc_box_kernel<256, uint32_t>), dim3((unsigned)blocks), dim3(256), lds, s, b);
+        } else {
+            if (wide) hipLaunchKernelGGL((mc_box_kernel<1024, double>), dim3((unsigned)blocks), dim3(1024), lds, s, b);
+            else hipLaunchKernelGGL((mc_box_kernel<256, double>), dim3((unsigned)blocks), dim3(256), lds, s, b);
+        }
     } else {
         if (a.clear_tally) {
             const hipError_t ef = launch_fill_f64(a.tally, nvox, 0.0, s);

@@ -301,9 +301,11 @@ class TissueGrid:
     def set_mc_launch(self, blocks_per_cu, rng_batch=32):
         self._chk(self.L.tam_set_mc_launch(self.h, int(blocks_per_cu), int(rng_batch)))
 
-    def set_mc_box(self, enable):
-        """Block-private LDS tally box on the beam axis on (default) or off."""
-        self._chk(self.L.tam_set_mc_box(self.h, 1 if enable else 0))
+    def set_mc_box(self, enable, fp64=False):
+        """Block-private LDS tally box on the beam axis on (default) or off;
+        fp64 keeps FP64 sums in it (half the voxels) instead of 32-bit fixed
+        point."""
+        self._chk(self.L.tam_set_mc_box(self.h, (2 if fp64 else 1) if enable else 0))
 
     def mc_box(self):
         """The LDS box of the last transport launch that placed one:
