@@ -247,12 +247,26 @@ __global__ __launch_bounds__(256) void conv_lds_kernel(ConvArgs a)
         // while the current one's MFMAs run
         bf16x8 afa[8], bfa[NT], afb[8], bfb[NT];
         load(0, afa, bfa);
+        if constexpr (CG == 1 && NT == 1) {
+            // unrolled: the accumulators stay in place (in a loop the compiler
+            // rotates them through one register quad every iteration)
+#pragma unroll
+            for (int kb = 0; kb < 14; kb += 2) {
+                load(kb + 1, afb, bfb);
+                mma(afa, bfa);
+                if (kb + 2 < 14) load(kb + 2, afa, bfa);
+                mma(afb, bfb);
+            }
+        } else {
+            // rolled: unrolled, the wider kernels grow past 160 VGPRs, and the
+            // channel-group loop copies the accumulators anyway
 #pragma unroll 1
-        for (int kb = 0; kb < 14; kb += 2) {
-            load(kb + 1, afb, bfb);
-            mma(afa, bfa);
-            if (kb + 2 < 14) load(kb + 2, afa, bfa);
-            mma(afb, bfb);
+            for (int kb = 0; kb < 14; kb += 2) {
+                load(kb + 1, afb, bfb);
+                mma(afa, bfa);
+                if (kb + 2 < 14) load(kb + 2, afa, bfa);
+                mma(afb, bfb);
+            }
         }
     }
     // lane holds voxels x0 + 4 (lane / 16) + j of row (y0 + t, z0 + w), channel 16 n + lane % 16
