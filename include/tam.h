@@ -183,13 +183,13 @@ int tam_mc_run(tam_ctx* ctx, int64_t npackets, uint64_t seed, uint32_t stream, i
  * the beam axis in LDS (the CU's LDS shared by the blocks per CU, at most the
  * device's per-block limit) and adds the box to the FP64 tally once at the
  * end, instead of one device atomic per voxel visit.  enable: 0 = off;
- * 1 = on with fixed-point sums in 32-bit integers (the default: twice the
- * voxels of FP64; each deposit rounded to 2^-22 of dx + dy + dz, and the sum
- * independent of the order of the adds; a slot hands every 2^30 units to the
- * tally, so it never overflows); 2 = on with FP64 sums.  With the box on,
- * blocks_per_cu 4 and 8 run as 1 and 2 blocks of 1024 lanes per CU: the same
- * lanes, one box per 1024 (34^2 x 35 voxels fixed point, 27^3 FP64, at 1 per
- * CU on MI355X).  Each transport launch places the cube's z range where the
+ * 1 = on with fixed-point sums in 16-bit integers, two voxels per word (the
+ * default: four times the voxels of FP64; each deposit rounded to 2^-13 of
+ * dx + dy + dz, and the sums independent of the order of the adds; a slot
+ * that wraps hands 2^16 units to the tally, so nothing is lost); 2 = on with
+ * FP64 sums.  With the box on, blocks_per_cu 4 and 8 run as 1 and 2 blocks of
+ * 1024 lanes per CU: the same lanes, one box per 1024 (42^2 x 46 voxels fixed
+ * point, 27^3 FP64, at 1 per CU on MI355X).  Each transport launch places the cube's z range where the
  * tally it is about to clear (the last run's path), over voxels with
  * mu_t > 0, sums highest inside the cube's x-y footprint (ties: the higher
  * range); with no path yet the cube starts at the top face.

@@ -38,12 +38,12 @@
 //    block therefore sums the visits to a cube of voxels on the beam axis in
 //    LDS (mc_box_kernel), placed in z where the last run's path was largest
 //    (box_plane_kernel, box_window_kernel), and adds the cube to the tally
-//    once at exit.  By default the box holds fixed-point sums in 32-bit
-//    integers (units of 2^-22 of dx + dy + dz): twice the voxels of an FP64
-//    box in the same LDS, integer LDS adds cost less than FP64 ones, and an
-//    integer sum does not depend on the order of the adds.  Each add that
-//    takes a slot past a multiple of 2^30 units moves 2^30 units to the
-//    device tally, so a slot never overflows.
+//    once at exit.  By default the box holds fixed-point sums in 16-bit
+//    integers, two voxels per 32-bit word (units of 2^-13 of dx + dy + dz):
+//    four times the voxels of an FP64 box in the same LDS, integer LDS adds
+//    cost less than FP64 ones, and an integer sum does not depend on the
+//    order of the adds.  Each add that wraps a 16-bit slot hands 2^16 units
+//    to the device tally (box_carry), so no sum is lost.
 //  * Transcendentals use the hardware approximations (v_log, v_sin/v_cos
 //    in revolutions, v_sqrt, v_rcp/v_rsq; ~1 ulp): they only shape sampled
 //    distances and directions, whose distributions the parity tests check.
@@ -101,8 +101,7 @@ __device__ __forceinline__ float face_dist(float o, float lo, float d, float u, 
 
 constexpr uint32_t kMaxEvents = 1u << 24;   // walk steps + random events of one packet
 constexpr uint32_t kChunk = 256;            // packets a wave takes from the queue at once
-constexpr uint32_t kBoxCarry = 1u << 30;    // a fixed-point box slot hands each 2^30 units to the tally
-constexpr float kBoxMaxAdd = 67108864.f;    // 2^26 units: a larger deposit goes to the tally directly
+constexpr int kBoxUnitBits = 13;            // fixed-point box unit: 2^-13 of dx + dy + dz
 
 enum : int { PEND_NONE = 0, PEND_LAUNCH = 1, PEND_INTERACT = 2, PEND_FRESNEL = 3 };
 
@@ -114,17 +113,19 @@ struct BoxGeom {
 };
 
 // A fixed-point box add whose carry test is still to run: the slot (-1:
-// none), its value before the add, the add, and the tally voxel.
+// none), its word's value before the add, the add, and the tally voxel.
 struct BoxAdd {
     int i;
     uint32_t old, q, v;
 };
 
 // Path length into the tally: voxels inside the block's LDS box are summed
-// there (BT = uint32_t: fixed point, BT = double: FP64), the rest go to the
-// FP64 device tally directly.  Returns 1 for a deposit into the box.  A
-// fixed-point add leaves its carry test in pa (box_carry), so that the LDS
-// return is awaited after the lane's other work.
+// there (BT = uint16_t: 16-bit fixed point, two slots per 32-bit word, slot
+// bi in half bi & 1; BT = double: FP64), the rest go to the FP64 device tally
+// directly.  Returns 1 for a deposit into the box.  A fixed-point add leaves
+// its carry test in pa (box_carry), so that the LDS return is awaited after
+// the lane's other work; a deposit of 2^16 units or more (a path over 8
+// (dx + dy + dz) in one visit) goes to the tally directly.
 template <bool BOX, class BT>
 __device__ __forceinline__ uint32_t tally_add(const McArgs& a, BT* box, const BoxGeom& g, uint32_t v, int ci,
                                               int cj, int ck, float l, BoxAdd& pa)
@@ -133,13 +134,15 @@ __device__ __forceinline__ uint32_t tally_add(const McArgs& a, BT* box, const Bo
         const unsigned bx = (unsigned)(ci - g.o[0]), by = (unsigned)(cj - g.o[1]), bz = (unsigned)(ck - g.o[2]);
         if (bx < (unsigned)g.n[0] && by < (unsigned)g.n[1] && bz < (unsigned)g.n[2]) {
             const unsigned bi = bx + (unsigned)g.n[0] * (by + (unsigned)g.n[1] * bz);
-            if constexpr (std::is_same<BT, uint32_t>::value) {
+            if constexpr (std::is_same<BT, uint16_t>::value) {
                 const float qf = rintf(l * a.box_scale);
-                if (qf < kBoxMaxAdd) {
+                if (qf < 65536.f) {
                     pa.i = (int)bi;
                     pa.q = (uint32_t)qf;
                     pa.v = v;
-                    pa.old = __hip_atomic_fetch_add(&box[bi], pa.q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    pa.old = __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(box) + (bi >> 1),
+                                                    pa.q << ((bi & 1u) * 16u), __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
                     return 1u;
                 }
             } else {
@@ -152,26 +155,34 @@ __device__ __forceinline__ uint32_t tally_add(const McArgs& a, BT* box, const Bo
     return 0u;
 }
 
-// The carry test of a fixed-point box add: the add that takes its slot past a
-// multiple of kBoxCarry units moves kBoxCarry units from the slot to the
-// tally.  Every crossing is seen by exactly one add, each takes back what it
-// moved, so the slot plus the carries always equals the sum of the adds; a
-// slot stays below 2^32 while fewer than three carries of one slot are in
-// flight (each needs ~1000 adds to that slot).
+// The carry test of a 16-bit box add.  A word holds the slots of two voxels
+// that are x-neighbours: the even slot in the low half, the next voxel in the
+// high half.  An add to the low half that wraps it carries 1 into the high
+// half: the low voxel's 2^16 units go to the tally, and 1 unit comes off the
+// high voxel there.  An add (to either half) that takes the word past 2^32
+// wraps the high half: the high voxel's 2^16 units go to the tally.  The
+// returned old value shows each wrap to exactly the add that made it, so the
+// slots plus these carries always equal the sums of the adds.
 template <class BT>
 __device__ __forceinline__ void box_carry(const McArgs& a, BT* box, BoxAdd& pa)
 {
-    if constexpr (std::is_same<BT, uint32_t>::value) {
-        if (pa.i >= 0 && ((pa.old + pa.q) ^ pa.old) >= kBoxCarry) {
-            __hip_atomic_fetch_sub(&box[pa.i], kBoxCarry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            unsafeAtomicAdd(&a.tally[pa.v], (double)kBoxCarry * a.box_quantum);
+    if constexpr (std::is_same<BT, uint16_t>::value) {
+        if (pa.i >= 0) {
+            const uint32_t sh = ((uint32_t)pa.i & 1u) * 16u;
+            const uint32_t vlo = sh ? pa.v - 1u : pa.v, vhi = vlo + 1u;
+            if (sh == 0u && (pa.old & 0xffffu) + pa.q >= 0x10000u) {
+                unsafeAtomicAdd(&a.tally[vlo], 65536.0 * a.box_quantum);
+                unsafeAtomicAdd(&a.tally[vhi], -a.box_quantum);
+            }
+            if ((uint64_t)pa.old + ((uint64_t)pa.q << sh) >= 0x100000000ull)
+                unsafeAtomicAdd(&a.tally[vhi], 65536.0 * a.box_quantum);
         }
         pa.i = -1;
     }
 }
 
 // A box slot's sum in path length
-__device__ __forceinline__ double box_path(uint32_t s, const McArgs& a) { return (double)s * a.box_quantum; }
+__device__ __forceinline__ double box_path(uint16_t s, const McArgs& a) { return (double)s * a.box_quantum; }
 __device__ __forceinline__ double box_path(double s, const McArgs&) { return s; }
 
 template <bool BOX, class BT>
@@ -446,7 +457,7 @@ __global__ __launch_bounds__(256) void mc_kernel(McArgs a)
 // device tally (one atomic per touched voxel) once every wave of the block
 // has drained the queue.  NT = 256, or 1024: one block of 16 waves per CU
 // holds one box of the CU's whole LDS instead of four quarter-size copies.
-// BT = uint32_t (fixed point, the default box) or double.
+// BT = uint16_t (16-bit fixed point, the default box) or double.
 template <int NT, class BT>
 __global__ __launch_bounds__(NT) void mc_box_kernel(McArgs a)
 {
@@ -673,7 +684,7 @@ hipError_t mc_prepare(int dev)
     hipError_t e = hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (e != hipSuccess || lds_blk <= 64 * 1024) return e;
     // allow the box kernels the device's whole per-block LDS (outside any graph capture)
-    const void* box_kernels[] = {(const void*)mc_box_kernel<256, uint32_t>, (const void*)mc_box_kernel<1024, uint32_t>,
+    const void* box_kernels[] = {(const void*)mc_box_kernel<256, uint16_t>, (const void*)mc_box_kernel<1024, uint16_t>,
                                  (const void*)mc_box_kernel<256, double>, (const void*)mc_box_kernel<1024, double>};
     for (const void* k : box_kernels) {
         e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds_blk);
@@ -710,27 +721,30 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
     if (need < blocks) blocks = need;
     // LDS tally box: the resident blocks of a CU share its LDS (160 KiB on
     // MI355X, also the most one block may take), 1 KiB per block kept for
-    // the kernel's own use: with 32-bit sums 34^2 x 35 voxels at 1 block per CU
-    // (1 or 4 blocks_per_cu), 27^3 at 2 (2 or 8), 23^3 at 3; with FP64 27^3,
-    // 21^3, 18^2 x 20.  The cube sits on the beam axis.  Its z range is where
+    // the kernel's own use: with 16-bit sums 42^2 x 46 voxels at 1 block per CU
+    // (1 or 4 blocks_per_cu), 34^2 x 34 at 2 (2 or 8), 28^2 x 34 at 3; with
+    // FP64 27^3, 21^3, 18^2 x 20.  The cube sits on the beam axis (its x edge
+    // even with 16-bit sums, so the two voxels of a word share a row).  Its z range is where
     // the last run's path was largest, or the top face (where most of the
     // path of a surface beam lies) before any run.  On the c1 skin cycles a
     // cube caught more deposits past the first cycle than flatter (d ~ s/2,
     // s/8) or taller (d ~ 2s) boxes of the same volume, and than a per-launch
     // choice among depths of 2-32 planes made from the last run's path.
-    // The box holds fixed-point sums (box_enable 1: twice the voxels of FP64)
-    // or FP64 ones (box_enable 2).  Fixed-point unit: 2^-22 of dx + dy + dz,
-    // so a deposit's rounding is ~1e-6 of a typical path in a voxel at most.
+    // The box holds 16-bit fixed-point sums (box_enable 1: four times the
+    // voxels of FP64) or FP64 ones (box_enable 2).  Fixed-point unit: 2^-13 of
+    // dx + dy + dz, so a deposit is rounded by at most ~1e-4 of a voxel-sized
+    // path, and a slot wraps (one carry) every ~40 typical deposits.
     const bool fx = a.box_enable == 1;
-    const long esz = fx ? 4 : 8;
+    const long esz = fx ? 2 : 8;
     McArgs b = a;
-    b.box_scale = (float)std::ldexp(1.0 / ((double)a.dx + a.dy + a.dz), 22);
+    b.box_scale = (float)std::ldexp(1.0 / ((double)a.dx + a.dy + a.dz), kBoxUnitBits);
     b.box_quantum = 1.0 / (double)b.box_scale;
     b.box_n[0] = b.box_n[1] = b.box_n[2] = 0;
     b.box_geom = nullptr;
     const long vox = std::min((long)lds_blk - 1024L, (long)lds_cu / bpc_l - 1024L) / esz;
     BoxPlace bp{};
     bp.sd = std::min((int)std::cbrt((double)vox), std::min(a.nx, a.ny));
+    if (fx && bp.sd > 1) bp.sd &= ~1;
     bp.d = bp.sd > 0 ? (int)std::min<long>(a.nz, vox / ((long)bp.sd * bp.sd)) : 0;
     if (a.box_enable && bp.sd >= 4 && bp.d >= 1) {
         bp.ox = std::max(0, std::min((int)std::lround(a.cx / a.dx) - bp.sd / 2, a.nx - bp.sd));
@@ -752,8 +766,8 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
             if (ef != hipSuccess) return ef;
         }
         if (fx) {
-            if (wide) hipLaunchKernelGGL((mc_box_kernel<1024, uint32_t>), dim3((unsigned)blocks), dim3(1024), lds, s, b);
-            else hipLaunchKernelGGL((mc_box_kernel<256, uint32_t>), dim3((unsigned)blocks), dim3(256), lds, s, b);
+            if (wide) hipLaunchKernelGGL((mc_box_kernel<1024, uint16_t>), dim3((unsigned)blocks), dim3(1024), lds, s, b);
+            else hipLaunchKernelGGL((mc_box_kernel<256, uint16_t>), dim3((unsigned)blocks), dim3(256), lds, s, b);
         } else {
             if (wide) hipLaunchKernelGGL((mc_box_kernel<1024, double>), dim3((unsigned)blocks), dim3(1024), lds, s, b);
             else hipLaunchKernelGGL((mc_box_kernel<256, double>), dim3((unsigned)blocks), dim3(256), lds, s, b);

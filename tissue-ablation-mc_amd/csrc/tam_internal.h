@@ -109,7 +109,7 @@ struct McArgs {
     // at launch (nullptr: npk and stream above)
     const int64_t* dyn_npk;
     const uint32_t* dyn_stream;
-    int box_enable;            // LDS tally box (single-grid launches): 0 off, 1 fixed-point sums, 2 FP64 sums
+    int box_enable;            // LDS tally box (single-grid launches): 0 off, 1 16-bit fixed-point sums, 2 FP64 sums
     float box_scale;           // fixed-point box: units per cm of path (set by launch_mc)
     double box_quantum;        // cm per unit, 1 / box_scale
     // block-private tally box in LDS (box_n[0] == 0: none): voxels
@@ -272,7 +272,7 @@ struct tam_ctx {
     long neblk;                 // slots: enough for the largest launch grid
     int mc_bpc;                 // transport launch: blocks per CU (autotuned)
     int mc_batch;               // transport launch: random-phase batch (autotuned)
-    int mc_box;                 // transport launch: LDS tally box off / fixed point / FP64 (0 / 1 / 2; on-off autotuned)
+    int mc_box;                 // transport launch: LDS tally box off / 16-bit fixed point / FP64 (0 / 1 / 2; on-off autotuned)
     float2* robin;
     double *kappa, *density, *heatcap, *coeff, *rhokap;
     float* mut;

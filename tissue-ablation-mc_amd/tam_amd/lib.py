@@ -303,8 +303,8 @@ class TissueGrid:
 
     def set_mc_box(self, enable, fp64=False):
         """Block-private LDS tally box on the beam axis on (default) or off;
-        fp64 keeps FP64 sums in it (half the voxels) instead of 32-bit fixed
-        point."""
+        fp64 keeps FP64 sums in it (a quarter of the voxels) instead of 16-bit
+        fixed point."""
         self._chk(self.L.tam_set_mc_box(self.h, (2 if fp64 else 1) if enable else 0))
 
     def mc_box(self):
