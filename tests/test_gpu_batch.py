@@ -1,8 +1,10 @@
 """Batched instances (tam_batch_*) against the same instances run one at a
 time: the transport of every instance in one launch gives each instance the
-integer fates of its own tam_mc_run on the same stream and the same tally up
-to FP64 summation order (the batched kernel's per-wave LDS boxes hold FP64
-sums, so the single runs use the FP64 box); batched heat steps are
+integer fates of its own tam_mc_run on the same stream, and the same tally up
+to the rounding of the deposits that either run sums in an LDS box (16-bit
+fixed point, 2^-13 (dx + dy + dz) per unit, at most half a unit per deposit;
+the batched kernel's per-wave boxes and the single run's CU-wide box cover
+different voxels): bound 1e-4 relative plus 16 units; batched heat steps are
 bit-identical with the single-instance step of the same launch shape."""
 import numpy as np
 import pytest
@@ -38,13 +40,14 @@ def test_batch_transport_matches_single_runs():
     b.close()
     for i, g in enumerate(gs):
         g.set_mc_launch(2, 24)
-        g.set_mc_box(True, fp64=True)
         s1 = g.mc_run(npk[i], seed=42, stream=10 + i)
         t1 = g.download(lib.FIELD_TALLY)
         for k in ("absorbed", "specular", "missed", "crossings", "interactions", "tally_adds"):
             assert getattr(st[i], k) == getattr(s1, k), (i, k)
         assert list(st[i].escaped) == list(s1.escaped)
-        np.testing.assert_allclose(tb[i], t1, rtol=1e-9, atol=1e-12)
+        p = probs[i].params
+        unit = 2.0 * (p.xmax / p.nx + p.ymax / p.ny + p.zmax / p.nz) / 2 ** 13
+        np.testing.assert_allclose(tb[i], t1, rtol=1e-4, atol=16 * unit)
         g.close()
 
 

@@ -123,6 +123,7 @@ McArgs mc_args(tam_ctx* c, int64_t npackets, uint64_t seed, uint32_t stream)
     a.box_enable = c->mc_box;
     a.box_psum = c->box_psum;              // the box follows the last run's path
     a.box_geom = c->box_geom;
+    mc_box_units(a);
     return a;
 }
 

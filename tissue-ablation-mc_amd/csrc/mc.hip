@@ -570,16 +570,18 @@ __global__ __launch_bounds__(256) void box_window_kernel(const double* __restric
 // queue until it drains, then moves on to the next instance (starting at its
 // own index, so the waves spread over the instances); each instance keeps its
 // own queue, tally and counters, and the instance's arguments stay uniform
-// over the wave.  With wbox > 0 each wave owns an LDS slice of wbox FP64
-// voxels: for every instance it works on (with the box enabled) it sums its
+// over the wave.  With wbox > 0 each wave owns an LDS slice of wbox 16-bit
+// fixed-point voxels: for every instance it works on (with the box enabled,
+// of either kind) it sums its
 // visits to a cube under the top face on that instance's beam axis there,
 // and adds the cube to the instance's tally when it moves on.  The waves of
 // a block work on different instances, so the slices are per wave and only
 // wave-level ordering is needed.
 __global__ __launch_bounds__(256) void mc_batch_kernel(const McArgs* __restrict__ tab, int n, int wbox)
 {
-    extern __shared__ double lds_slices[];
-    double* box = lds_slices + (size_t)(threadIdx.x >> 6) * (size_t)wbox;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_slices[];
+    // wbox is a multiple of 8, so every slice starts on a 16-byte boundary
+    uint16_t* box = reinterpret_cast<uint16_t*>(lds_slices) + (size_t)(threadIdx.x >> 6) * (size_t)wbox;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     for (int k = 0; k < n; ++k) {
@@ -593,6 +595,7 @@ __global__ __launch_bounds__(256) void mc_batch_kernel(const McArgs* __restrict_
             while ((long)(sd + 1) * (sd + 1) * (sd + 1) <= wbox) ++sd;
             while (sd > 1 && (long)sd * sd * sd > wbox) --sd;
             sd = min(sd, min(a.nx, a.ny));
+            if (sd > 1) sd &= ~1;                          // the two voxels of a word share a row
             const int d = min(a.nz, wbox / (sd * sd));
             g.n[0] = g.n[1] = sd;
             g.n[2] = d;
@@ -605,15 +608,15 @@ __global__ __launch_bounds__(256) void mc_batch_kernel(const McArgs* __restrict_
             transport<false, double>(a, nullptr, g);
             continue;
         }
-        for (int q = lane; q < nb; q += 64) box[q] = 0.0;
+        for (int q = lane; q < nb; q += 64) box[q] = 0;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");   // the zeros land before any add
         __builtin_amdgcn_wave_barrier();
-        transport<true, double>(a, box, g);
+        transport<true, uint16_t>(a, box, g);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");   // every lane's adds land before the flush
         __builtin_amdgcn_wave_barrier();
         const uint32_t nxy = (uint32_t)a.nx * (uint32_t)a.ny;
         for (int q = lane; q < nb; q += 64) {
-            const double s = box[q];
+            const double s = box_path(box[q], a);
             if (s != 0.0) {
                 const int bx = q % g.n[0], by = (q / g.n[0]) % g.n[1], bz = q / (g.n[0] * g.n[1]);
                 const uint32_t v = (uint32_t)(g.o[0] + bx) + (uint32_t)a.nx * (uint32_t)(g.o[1] + by) +
@@ -672,10 +675,17 @@ hipError_t launch_mc_batch(const McArgs* dtab, int n, long total_npk, int blocks
         lds_blk <= 0)
         lds_blk = 64 * 1024;
     const long per_block = std::min((long)lds_blk - 1024L, (long)lds_cu / bpc - 1024L);
-    const int wbox = per_block > 0 ? (int)(per_block / 4 / 8) : 0;
-    const size_t lds = sizeof(double) * 4 * (size_t)wbox;
+    // 16-bit slots, a multiple of 8 per wave
+    const int wbox = per_block > 0 ? (int)(per_block / 4 / 2) & ~7 : 0;
+    const size_t lds = sizeof(uint16_t) * 4 * (size_t)wbox;
     hipLaunchKernelGGL(mc_batch_kernel, dim3((unsigned)blocks), dim3(256), lds, s, dtab, n, wbox);
     return hipGetLastError();
+}
+
+void mc_box_units(McArgs& a)
+{
+    a.box_scale = (float)std::ldexp(1.0 / ((double)a.dx + a.dy + a.dz), kBoxUnitBits);
+    a.box_quantum = 1.0 / (double)a.box_scale;
 }
 
 hipError_t mc_prepare(int dev)
@@ -737,8 +747,7 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
     const bool fx = a.box_enable == 1;
     const long esz = fx ? 2 : 8;
     McArgs b = a;
-    b.box_scale = (float)std::ldexp(1.0 / ((double)a.dx + a.dy + a.dz), kBoxUnitBits);
-    b.box_quantum = 1.0 / (double)b.box_scale;
+    mc_box_units(b);
     b.box_n[0] = b.box_n[1] = b.box_n[2] = 0;
     b.box_geom = nullptr;
     const long vox = std::min((long)lds_blk - 1024L, (long)lds_cu / bpc_l - 1024L) / esz;

@@ -326,6 +326,8 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s);
 // once per context, before any capture: lets the LDS-box kernel take the
 // device's whole per-block LDS
 hipError_t mc_prepare(int dev);
+// the fixed-point LDS box's unit for a's voxel size (box_scale, box_quantum)
+void mc_box_units(McArgs& a);
 // all instances of a device table in one launch (see mc_batch_kernel)
 hipError_t launch_mc_batch(const McArgs* dtab, int n, long total_npk, int blocks_per_cu, hipStream_t s);
 // `count` FP64 atomic adds (whole waves) to pseudo-random slots of t[0, slots)
