@@ -177,8 +177,10 @@ int tam_mc_run(tam_ctx* ctx, int64_t npackets, uint64_t seed, uint32_t stream, i
  * wave must wait for random numbers before the wave draws them together.
  * tam_tune_mc times candidate pairs on a sample of `sample` packets and keeps
  * the fastest (the tally is cleared afterwards), with the LDS tally box on
- * and off.  None of them changes results beyond FP64 summation order:
- * packet p always draws from its own stream.
+ * and off; near-ties go to the CU-wide box (4 blocks per CU, within 10%) and
+ * to a random-phase batch of 32 (within 3%), the fastest once a crater
+ * forms.  None of them changes results beyond FP64 summation order: packet p
+ * always draws from its own stream.
  * tam_set_mc_box: each block sums the path it lays in a cube of voxels on
  * the beam axis in LDS (the CU's LDS shared by the blocks per CU, at most the
  * device's per-block limit) and adds the box to the FP64 tally once at the

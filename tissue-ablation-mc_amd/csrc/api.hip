@@ -46,6 +46,8 @@ constexpr int kMaxDevices = 64;
 // tam_tune_mc keeps the CU-wide LDS tally box when it times within this
 // fraction of the fastest candidate
 constexpr double kWideBoxMargin = 0.10;
+// ... and a random-phase batch of 32 when it times within this fraction of the fastest batch
+constexpr double kBatchMargin = 0.03;
 std::mutex scratch_mu;
 double* scratch_dev[kMaxDevices] = {};
 
@@ -1316,6 +1318,18 @@ int tam_tune_mc(tam_ctx* c, int64_t sample, int32_t* blocks_per_cu, int32_t* rng
             bx = box_on;
             best = wide;
         }
+    }
+    // With the CU-wide box, a random-phase batch of 32 is kept when it times
+    // within kBatchMargin of the fastest batch: on fresh tissue the batches
+    // 8-48 tie within the timing noise, while in the c1 crater phase 24-32
+    // run fastest (318 ms per 1e8 packets against 348 ms at 8 and 353 at 48).
+    if (bx && bb == 4 && bt != 32) {
+        for (size_t q = 0; q < rates.size(); ++q)
+            if (rates[q].box && rates[q].bpc == 4 && rates[q].batch == 32 &&
+                rates[q].rate >= (1.0 - kBatchMargin) * best) {
+                bt = 32;
+                best = rates[q].rate;
+            }
     }
     c->mc_bpc = bb;
     c->mc_batch = bt;
