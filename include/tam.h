@@ -300,9 +300,10 @@ int tam_synchronize(tam_ctx* ctx);
  * tam_batch_mc_run: the transport of all instances in one launch (instance
  *   i: npackets[i] packets on Philox stream stream0 + i of key seed, as
  *   tam_mc_run would give); stats[i] per instance when stats is not NULL.
- *   Instances with the LDS tally box on (tam_set_mc_box) get a cube under
- *   their top face on the beam axis per wave (a quarter of the block's
- *   LDS), not placed from the last run's path.
+ *   Instances with the LDS tally box on (tam_set_mc_box, either kind) get a
+ *   cube under their top face on the beam axis per wave (a quarter of the
+ *   block's LDS, 16-bit fixed-point sums), not placed from the last run's
+ *   path.
  * tam_batch_heat_run: `loops` steps of every instance, one launch per step
  *   or, with a two-step shape in class mode, per pair of steps (laser on;
  *   the source from each instance's last tam_normalise).
