@@ -755,7 +755,11 @@ hipError_t launch_mc(const McArgs& a, hipStream_t s)
     bp.sd = std::min((int)std::cbrt((double)vox), std::min(a.nx, a.ny));
     if (fx && bp.sd > 1) bp.sd &= ~1;
     bp.d = bp.sd > 0 ? (int)std::min<long>(a.nz, vox / ((long)bp.sd * bp.sd)) : 0;
-    if (a.box_enable && bp.sd >= 4 && bp.d >= 1) {
+    // A launch with fewer packets than a quarter of the lanes (the reference's
+    // own run: 125 000 packets per cycle of a few deposits each) goes without
+    // the box: clearing and flushing the box would cost more than it saves.
+    const bool few = (double)a.npk < 0.25 * (double)ncu * bpc_l * lanes;
+    if (a.box_enable && !few && bp.sd >= 4 && bp.d >= 1) {
         bp.ox = std::max(0, std::min((int)std::lround(a.cx / a.dx) - bp.sd / 2, a.nx - bp.sd));
         bp.oy = std::max(0, std::min((int)std::lround(a.cy / a.dy) - bp.sd / 2, a.ny - bp.sd));
         b.box_o[0] = bp.ox;
