@@ -370,67 +370,76 @@ __device__ __forceinline__ void transport(const McArgs& a, BT* box, const BoxGeo
                     }
                 }
             }
-        } else if (pend == PEND_NONE) {
-            // ================= walk phase: one face crossing per lane ======
-            if (++nev > kMaxEvents) {
-                // a packet that neither escapes nor is absorbed within
-                // kMaxEvents steps (e.g. total internal reflection in a
-                // non-absorbing slab) is abandoned, so every lane's loop ends
-                ++c_kill;
-                have = false;
-                continue;
-            }
-            int axis = 0;
-            float tn = Tx;
-            if (Ty < tn) { tn = Ty; axis = 1; }
-            if (Tz < tn) { tn = Tz; axis = 2; }
-            const float s = fmaxf(tn - t, 0.f);
-            if (mu * s < tau) {
-                tau -= mu * s;
-                lacc += s;
-                ++c_cross;
-                // mu_t = 0 (ablated): nothing is absorbed, and the atomic
-                // unit's throughput is what bounds this kernel
-                if (mu > 0.f) {
-                    c_box += tally_add<BOX, BT>(a, box, g, v, ci, cj, ck, lacc, pa);
-                    ++c_tally;
-                }
-                lacc = 0.f;
-                t = tn;
-                bool out = false;
-                int face = 0;
-                if (axis == 0) {
-                    Tx += a.dx * fabsf(iux);
-                    if (ux > 0.f) { ++ci; ++v; out = ci >= a.nx; face = TAM_FACE_XHI; }
-                    else          { --ci; --v; out = ci < 0;     face = TAM_FACE_XLO; }
-                } else if (axis == 1) {
-                    Ty += a.dy * fabsf(iuy);
-                    if (uy > 0.f) { ++cj; v += a.nx; out = cj >= a.ny; face = TAM_FACE_YHI; }
-                    else          { --cj; v -= a.nx; out = cj < 0;     face = TAM_FACE_YLO; }
-                } else {
-                    const bool top = uz > 0.f;
-                    if (top ? (ck == a.nz - 1) : (ck == 0)) {
-                        // z face of the grid: Fresnel test where the index changes
-                        nout = top ? a.n_zhi : a.n_zlo;
-                        nin = a.mtab[a.mat[v]].n;
-                        if (nout > 0.f && nout != nin) pend = PEND_FRESNEL;
-                        else { out = true; face = top ? TAM_FACE_ZHI : TAM_FACE_ZLO; }
+        } else {
+            // ================= walk phase: face crossings, one per lane and
+            // pass, until `batch` lanes wait for a random event or a packet
+            // (or every lane does).  Staying in this loop keeps the walk
+            // state in place between crossings.
+            uint64_t sm;
+            do {
+                if (pend == PEND_NONE && have) {
+                    if (++nev > kMaxEvents) {
+                        // a packet that neither escapes nor is absorbed within
+                        // kMaxEvents steps (e.g. total internal reflection in a
+                        // non-absorbing slab) is abandoned, so every lane's loop ends
+                        ++c_kill;
+                        have = false;
                     } else {
-                        Tz += a.dz * fabsf(iuz);
-                        if (top) { ++ck; v += nxy; } else { --ck; v -= nxy; }
+                        int axis = 0;
+                        float tn = Tx;
+                        if (Ty < tn) { tn = Ty; axis = 1; }
+                        if (Tz < tn) { tn = Tz; axis = 2; }
+                        const float s = fmaxf(tn - t, 0.f);
+                        if (mu * s < tau) {
+                            tau -= mu * s;
+                            lacc += s;
+                            ++c_cross;
+                            // mu_t = 0 (ablated): nothing is absorbed, and the atomic
+                            // unit's throughput is what bounds this kernel
+                            if (mu > 0.f) {
+                                c_box += tally_add<BOX, BT>(a, box, g, v, ci, cj, ck, lacc, pa);
+                                ++c_tally;
+                            }
+                            lacc = 0.f;
+                            t = tn;
+                            bool out = false;
+                            int face = 0;
+                            if (axis == 0) {
+                                Tx += a.dx * fabsf(iux);
+                                if (ux > 0.f) { ++ci; ++v; out = ci >= a.nx; face = TAM_FACE_XHI; }
+                                else          { --ci; --v; out = ci < 0;     face = TAM_FACE_XLO; }
+                            } else if (axis == 1) {
+                                Ty += a.dy * fabsf(iuy);
+                                if (uy > 0.f) { ++cj; v += a.nx; out = cj >= a.ny; face = TAM_FACE_YHI; }
+                                else          { --cj; v -= a.nx; out = cj < 0;     face = TAM_FACE_YLO; }
+                            } else {
+                                const bool top = uz > 0.f;
+                                if (top ? (ck == a.nz - 1) : (ck == 0)) {
+                                    // z face of the grid: Fresnel test where the index changes
+                                    nout = top ? a.n_zhi : a.n_zlo;
+                                    nin = a.mtab[a.mat[v]].n;
+                                    if (nout > 0.f && nout != nin) pend = PEND_FRESNEL;
+                                    else { out = true; face = top ? TAM_FACE_ZHI : TAM_FACE_ZLO; }
+                                } else {
+                                    Tz += a.dz * fabsf(iuz);
+                                    if (top) { ++ck; v += nxy; } else { --ck; v -= nxy; }
+                                }
+                            }
+                            if (out) {
+                                ++c_esc[face];
+                                have = false;
+                            } else if (pend == PEND_NONE) {
+                                mu = a.mut[v];
+                            }
+                            box_carry<BT>(a, box, pa);
+                        } else {
+                            pend = PEND_INTERACT;
+                            mat = a.mat[v];
+                        }
                     }
                 }
-                if (out) {
-                    ++c_esc[face];
-                    have = false;
-                } else if (pend == PEND_NONE) {
-                    mu = a.mut[v];
-                }
-                box_carry<BT>(a, box, pa);
-            } else {
-                pend = PEND_INTERACT;
-                mat = a.mat[v];
-            }
+                sm = ballot(pend != PEND_NONE || !have);
+            } while ((uint32_t)__popcll(sm) < batch && sm != am);
         }
     }
 
