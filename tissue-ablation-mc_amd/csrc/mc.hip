@@ -275,29 +275,28 @@ __device__ __forceinline__ void transport(const McArgs& a, BT* box, const BoxGeo
                     cj = (int)floorf(y * idy_);
                     ck = a.nz - 1;
                     const bool inside = x >= 0.f && y >= 0.f && ci >= 0 && ci < a.nx && cj >= 0 && cj < a.ny;
-                    if (!inside) {
-                        ++c_miss;
-                        have = false;
-                    } else {
-                        v = (uint32_t)ci + (uint32_t)a.nx * (uint32_t)cj + nxy * (uint32_t)ck;
-                        if (u2 < a.mtab[a.mat[v]].rsp_top) {
-                            ++c_spec;
-                            have = false;
-                        } else {
-                            const float xl = (float)ci * a.dx, yl = (float)cj * a.dy;
-                            ox = fminf(fmaxf(x, xl), xl + a.dx);
-                            oy = fminf(fmaxf(y, yl), yl + a.dy);
-                            oz = ztop;
-                            ux = 0.f; uy = 0.f; uz = -1.f;
-                            iux = 0.f; iuy = 0.f; iuz = -1.f;
-                            t = 0.f;
-                            Tx = 3.0e38f; Ty = 3.0e38f; Tz = a.dz;
-                            mu = a.mut[v];
-                            tau = neg_ln(u3);
-                            lacc = 0.f;
-                        }
-                    }
-                } else if (kind == PEND_INTERACT) {
+                    // Every launch lane sets the whole walk state: a packet
+                    // that misses the grid or is reflected ends here, and its
+                    // state is never read (voxel 0 stands in for its loads).
+                    // One straight path keeps the state in its registers.
+                    v = inside ? (uint32_t)ci + (uint32_t)a.nx * (uint32_t)cj + nxy * (uint32_t)ck : 0u;
+                    mu = a.mut[v];
+                    const bool spec = inside && u2 < a.mtab[a.mat[v]].rsp_top;
+                    c_miss += inside ? 0u : 1u;
+                    c_spec += spec ? 1u : 0u;
+                    have = inside && !spec;
+                    const float xl = (float)ci * a.dx, yl = (float)cj * a.dy;
+                    ox = fminf(fmaxf(x, xl), xl + a.dx);
+                    oy = fminf(fmaxf(y, yl), yl + a.dy);
+                    oz = ztop;
+                    ux = 0.f; uy = 0.f; uz = -1.f;
+                    iux = 0.f; iuy = 0.f; iuz = -1.f;
+                    t = 0.f;
+                    Tx = 3.0e38f; Ty = 3.0e38f; Tz = a.dz;
+                    tau = neg_ln(u3);
+                    lacc = 0.f;
+                }
+                if (kind == PEND_INTERACT) {
                     // interaction inside voxel v, tau / mu beyond the current point
                     const float s = tau * rcp(mu);
                     lacc += s;
@@ -349,7 +348,8 @@ __device__ __forceinline__ void transport(const McArgs& a, BT* box, const BoxGeo
                         Tz = face_dist(oz, zl, a.dz, uz, iuz);
                         tau = neg_ln(u01(r.w));
                     }
-                } else {
+                }
+                if (kind == PEND_FRESNEL) {
                     // Fresnel test at a z face of the grid (the crossing is tallied)
                     const bool top = uz > 0.f;
                     if (u01(r.x) < fresnel(nin, nout, fabsf(uz))) {
@@ -370,13 +370,14 @@ __device__ __forceinline__ void transport(const McArgs& a, BT* box, const BoxGeo
                     }
                 }
             }
-        } else {
-            // ================= walk phase: face crossings, one per lane and
-            // pass, until `batch` lanes wait for a random event or a packet
-            // (or every lane does).  Staying in this loop keeps the walk
-            // state in place between crossings.
-            uint64_t sm;
-            do {
+        }
+        // ================= walk phase: face crossings, one per lane and
+        // pass, until `batch` lanes wait for a random event or a packet (or
+        // every lane does).  Staying in this loop keeps the walk state in
+        // place between crossings.
+        {
+            uint64_t sm = ballot(pend != PEND_NONE || !have);
+            while ((uint32_t)__popcll(sm) < batch && sm != am) {
                 if (pend == PEND_NONE && have) {
                     if (++nev > kMaxEvents) {
                         // a packet that neither escapes nor is absorbed within
@@ -439,7 +440,7 @@ __device__ __forceinline__ void transport(const McArgs& a, BT* box, const BoxGeo
                     }
                 }
                 sm = ballot(pend != PEND_NONE || !have);
-            } while ((uint32_t)__popcll(sm) < batch && sm != am);
+            }
         }
     }
 
