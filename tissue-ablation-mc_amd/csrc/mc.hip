@@ -33,8 +33,8 @@
 //    voxel and written with one FP64 atomic add per voxel visit (at exit or
 //    absorption).  The absorbed fraction per voxel is mu_a * L.  Voxels with
 //    mu_t = 0 absorb nothing and are crossed without a tally.  The global
-//    atomic rate (~2.4e10/s on MI355X whatever the address spread) is the
-//    bound of this kernel: without the tally it runs ~4.5x faster.  Each
+//    atomic rate (~2.4e10/s on MI355X whatever the address spread) bounds
+//    a kernel that sends every visit to the device tally.  Each
 //    block therefore sums the visits to a cube of voxels on the beam axis in
 //    LDS (mc_box_kernel), placed in z where the last run's path was largest
 //    (box_plane_kernel, box_window_kernel), and adds the cube to the tally
