@@ -128,6 +128,9 @@ def test_mc_launch_shape_does_not_change_results():
         g.set_mc_launch(bpc, batch)
         g.set_mc_box(box > 0, fp64=box == 2)
         st = g.mc_run(prob.packets, seed=77, stream=3)
+        # 300 000 packets clear the short-launch threshold (a quarter of the
+        # launch's lanes) at every shape here, so a box case sums in its box
+        assert (st.box_adds > 0) == (box > 0), (bpc, box, st.box_adds)
         res.append((box, st, g.download(lib.FIELD_TALLY)))
     b, t, rate = g.tune_mc(100_000)
     assert 1 <= b <= 64 and 1 <= t <= 64 and rate > 0
