@@ -147,6 +147,33 @@ def test_mc_launch_shape_does_not_change_results():
             np.testing.assert_allclose(tl, t0, rtol=1e-9, atol=1e-12)
 
 
+def test_short_launch_runs_without_the_box():
+    """A launch with fewer packets than a quarter of its lanes (4 blocks per
+    CU: one 1024-lane block per CU) runs without the LDS tally box even when
+    the box is on, since clearing and flushing it would cost more than it
+    saves (the reference's own run: 125 000 packets per cycle).  Its fates and
+    tally equal those of the same launch with the box off; a launch of more
+    packets at the same shape does use the box."""
+    prob = _two_layer_problem(n=32, packets=20_000)
+    g = prob.grid()
+    g.set_mc_launch(4, 32)
+    res = []
+    for box in (False, True):
+        _print(f"case: 20 000 packets, 4 blocks per CU, box {'on' if box else 'off'}")
+        g.set_mc_box(box)
+        st = g.mc_run(prob.packets, seed=5, stream=1)
+        res.append((st, g.download(lib.FIELD_TALLY)))
+    (s0, t0), (s1, t1) = res
+    assert s0.box_adds == 0 and s1.box_adds == 0
+    for k in ("absorbed", "specular", "missed", "crossings", "interactions", "killed", "tally_adds"):
+        assert getattr(s1, k) == getattr(s0, k), k
+    np.testing.assert_allclose(t1, t0, rtol=1e-9, atol=1e-12)
+    _print("case: 300 000 packets, 4 blocks per CU, box on")
+    st = g.mc_run(300_000, seed=5, stream=1)
+    assert st.box_adds > 0
+    g.close()
+
+
 def test_fixed_point_box_matches_fp64_box_at_c1_scale():
     """The 16-bit fixed-point LDS tally box against the FP64 one on the
     three-layer skin at 200^3 with 1e8 packets (the transport of one c1
